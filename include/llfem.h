@@ -1,0 +1,165 @@
+/*
+ * libllfem -- MI355X (gfx950) operator-assembly library for LowLevelFEM.jl.
+ *
+ * C ABI only: plain pointers, sizes and POD structs; no C++ or torch types.
+ * Every host array is owned by the caller; the library never frees caller
+ * memory.  Device memory is owned by the context / mesh handles.  Functions
+ * return LLFEM_OK (0) or a negative LLFEM_E* code; llfem_last_error() then
+ * holds a message for the calling thread (a Julia shim rethrows it with
+ * error(msg), keeping the reference's messages where it has one).
+ *
+ * Indices at this boundary are int64.  `one_based` selects Julia's 1-based
+ * node tags / DoF numbers (dof = pdim*(tag-1)+c, src/linear.jl:309-312);
+ * 0 selects 0-based.  Output CSC arrays use the same base.
+ *
+ * Each entry point names the reference function it replaces.  Reference
+ * citations are to LowLevelFEM.jl 1.13.31.
+ */
+#ifndef LLFEM_H
+#define LLFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define LLFEM_API __attribute__((visibility("default")))
+#else
+#define LLFEM_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct llfem_ctx llfem_ctx;
+typedef struct llfem_mesh llfem_mesh;
+
+enum {
+  LLFEM_OK = 0,
+  LLFEM_EINVAL = -1,   /* bad argument / size mismatch (src/multifield.jl:2784-2792) */
+  LLFEM_EHIP = -2,     /* HIP runtime failure */
+  LLFEM_ENOMEM = -3,   /* device allocation failed or HBM budget exceeded */
+  LLFEM_EGEOM = -4,    /* singular geometry at an integration point (src/multifield.jl:1737-1739) */
+  LLFEM_EPATTERN = -5, /* row missing from the CSC pattern (src/multifield.jl:2282) */
+  LLFEM_ENOCONV = -6,  /* iterative solver hit maxiter */
+  LLFEM_EUNSUP = -7    /* element type / option not supported */
+};
+
+enum { LLFEM_OP_STIFFNESS = 0, LLFEM_OP_MASS = 1 };
+
+/* Arithmetic of the element integration.  The global CSC summation is FP64
+ * in every mode. */
+enum {
+  LLFEM_FP64 = 0,               /* FP64 throughout (the parity path) */
+  LLFEM_FP32_INT_FP64_SUM = 1,  /* FP32 element integration, FP64 summation */
+  LLFEM_BF16X3 = 2              /* high-order B'DB as split BF16 MFMA, FP32 accumulate */
+};
+
+enum {
+  LLFEM_PCG_JACOBI_FP64 = 0,    /* Jacobi-PCG in FP64 */
+  LLFEM_PCG_MIXED_REFINE = 1    /* FP32 Jacobi-PCG inner solves + FP64 residual refinement */
+};
+
+/* One operator request.  The tables are what gmsh hands the reference:
+ * getIntegrationPoints / getBasisFunctions(..., "Lagrange"|"GradLagrange")
+ * in gmsh's flattened layout (src/linear.jl:235-239). */
+typedef struct {
+  int32_t op;              /* LLFEM_OP_* */
+  int32_t q;               /* number of integration points */
+  const double* weights;   /* [q] */
+  const double* N_ref;     /* [q][npe]      (mass, traction) */
+  const double* dN_ref;    /* [q][npe][3]   (stiffness, geometry) */
+  const double* D;         /* 6x6 row-major in SymGrad row order
+                              [xx,yy,zz,xy,xz,yz] (src/multifield.jl:635-657),
+                              or NULL for Hooke's law from E, nu
+                              (constitutive_matrix, src/linear.jl:154-165) */
+  double E, nu, rho;
+  int32_t lumped;          /* mass: row-sum diagonal (src/linear.jl:1999-2001) */
+  int32_t precision;       /* LLFEM_FP64 | LLFEM_FP32_INT_FP64_SUM | LLFEM_BF16X3 */
+} llfem_op_desc;
+
+typedef struct {
+  double kernel_ms;        /* device time of the assembly kernels (HIP events, library stream) */
+  double main_kernel_ms;   /* device time of the dominant kernel alone */
+  double total_ms;         /* host wall time of the call */
+  int64_t nnz;             /* structural non-zeros written */
+  int64_t elements;
+  double alg_bytes;        /* algorithmic HBM bytes of the call */
+  int64_t iterations;      /* solver: total (inner) CG iterations */
+  int64_t refinements;     /* solver: FP64 refinement steps */
+  double relres;           /* solver: ||(K u - f)_free|| / ||f_free|| */
+} llfem_stats;
+
+/* ---- context ---------------------------------------------------------- */
+/* One context per host thread; calls on a context are serialised on its
+ * HIP stream.  hbm_budget_bytes = 0 means no budget. */
+LLFEM_API int llfem_ctx_create(int device, size_t hbm_budget_bytes, llfem_ctx** out);
+LLFEM_API int llfem_ctx_destroy(llfem_ctx* ctx);
+LLFEM_API int llfem_ctx_synchronize(llfem_ctx* ctx);
+LLFEM_API size_t llfem_ctx_bytes_in_use(const llfem_ctx* ctx);
+LLFEM_API const char* llfem_last_error(void);
+
+/* ---- mesh --------------------------------------------------------------
+ * Replaces the gmsh queries + pattern construction of the reference:
+ * dense_node_coordinates (src/multifield.jl:1612-1648), getElementsByType,
+ * and build_csc_pattern (src/multifield.jl:2083-2250).  Uploads coordinates
+ * (xyz: [N][3], node-tag order) and connectivity ([E][npe], element-major),
+ * then builds on the device, once per mesh: the structural CSC pattern
+ * (node adjacency expanded to pdim x pdim blocks, columns node-major /
+ * component-minor, rows sorted) and the element->nnz map that replaces
+ * find_csc_position (src/multifield.jl:2260-2283). */
+LLFEM_API int llfem_mesh_create(llfem_ctx* ctx, const double* xyz, int64_t N, int32_t elem_type,
+                      int32_t npe, const int64_t* conn, int64_t E, int32_t pdim,
+                      int32_t one_based, llfem_mesh** out);
+LLFEM_API int llfem_mesh_destroy(llfem_mesh* mesh);
+
+/* Structural pattern of an operator (LLFEM_OP_STIFFNESS: the K pattern of
+ * build_csc_pattern; LLFEM_OP_MASS: node adjacency x I_pdim, the pattern
+ * massMatrix returns after dropzeros!, src/linear.jl:2002).  Two-phase:
+ * pass colptr = rowval = NULL to query nnz. */
+LLFEM_API int llfem_mesh_pattern(llfem_mesh* mesh, int32_t op, int32_t one_based, int64_t* nnz,
+                       int64_t* colptr, int64_t* rowval);
+
+/* ---- assembly ----------------------------------------------------------
+ * LLFEM_OP_STIFFNESS replaces stiffnessMatrix(problem) (src/linear.jl:70-87,
+ * stiffnessMatrixSolid :190-330) and the DSL
+ * ∫(SymGrad(Pu)⋅D⋅SymGrad(Pu); csc_matrix=K) path (src/multifield.jl:5295,
+ * assemble_operator :2630-3280).  LLFEM_OP_MASS replaces
+ * massMatrix(problem; lumped) (src/linear.jl:1843-2004).
+ * The values stay resident on the device (reused by llfem_solve); when
+ * nzval_out != NULL they are also copied to the host, in the order of
+ * llfem_mesh_pattern(op) (for lumped mass: the pdim*N diagonal). */
+LLFEM_API int llfem_assemble(llfem_mesh* mesh, const llfem_op_desc* desc, double* nzval_out,
+                   llfem_stats* stats);
+
+/* ---- load vector -------------------------------------------------------
+ * Replaces the traction branch of loadVector(problem, [load(name, fx,fy,fz)])
+ * (src/linear.jl:3088-3434): f_a += sum_q N_a t |t1 x t2| w_q over the face
+ * elements (face_conn [F][nf]); f_out is [pdim*N], zero-initialised here. */
+LLFEM_API int llfem_load_traction(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
+                        int64_t F, int32_t one_based, int32_t q, const double* weights,
+                        const double* N_ref, const double* dN_ref, const double traction[3],
+                        double* f_out);
+
+/* ---- solve -------------------------------------------------------------
+ * Replaces solveDisplacement(K, f; support, iterative=true, reltol, maxiter,
+ * preconditioner) (src/linear.jl:3711-3743) on the stiffness most recently
+ * assembled on `mesh`: u_fixed on the constrained DoFs, rhs = f - K u_fixed
+ * on the free DoFs, K_ff u_free = rhs by Jacobi-PCG.  fixed[] lists the
+ * constrained DoFs (constrainedDoFs, src/general.jl:5289-5322) and
+ * u_fixed[] their values (NULL = zeros).  u_out is [pdim*N]. */
+LLFEM_API int llfem_solve(llfem_mesh* mesh, const double* f, const int64_t* fixed, int64_t nfixed,
+                int32_t one_based, const double* u_fixed, int32_t method, double reltol,
+                int64_t maxiter, double* u_out, llfem_stats* stats);
+
+/* Same solve for a caller-supplied symmetric CSC matrix (the SystemMatrix
+ * the reference passes in): ndof x ndof, colptr [ndof+1], rowval/nzval [nnz]. */
+LLFEM_API int llfem_solve_csc(llfem_ctx* ctx, int64_t ndof, const int64_t* colptr, const int64_t* rowval,
+                    const double* nzval, int32_t one_based, const double* f,
+                    const int64_t* fixed, int64_t nfixed, const double* u_fixed, int32_t method,
+                    double reltol, int64_t maxiter, double* u_out, llfem_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLFEM_H */

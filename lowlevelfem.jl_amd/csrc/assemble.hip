@@ -1,0 +1,572 @@
+// Operator assembly: element geometry + node-panel accumulation.
+//
+// Reference algorithm: K = sum_e P_e^T Ke P_e with
+//   Ke = sum_q B_q^T D B_q |det J_q| w_q        (src/multifield.jl:1857-2052,
+//                                                 src/linear.jl:272-307)
+// scattered into the CSC pattern block by block (src/multifield.jl:2465-2493).
+//
+// MI355X design (owner computes, no atomics on HBM, no staging of Ke):
+//   * In node-major CSC the three columns of node j are one contiguous run of
+//     9*R_j doubles ("panel": [column c][row slot s][row comp d]) where R_j is
+//     the number of adjacent nodes.  Each panel is owned by one wavefront.
+//   * The wavefront walks j's incidences (element e, local index a) in
+//     ascending element order; lane b computes the 3x3 block Ke[(b,:),(a,:)]
+//     and adds it to LDS at the slot given by the element->nnz map.  Within
+//     one step every lane of a node hits a different slot and LDS executes a
+//     wave's instructions in order, so the sums are deterministic and
+//     conflict-free (ds_add_f64 is used as a plain add).
+//   * The finished panel is written once, coalesced.  HBM traffic is the
+//     nzval write plus index/geometry reads (no read-modify-write of nzval).
+//   * Nodes are processed in classes of equal incidence count so the
+//     incidence loop never diverges inside a wave; 64/npe nodes share a wave.
+//   * Affine elements (constant Jacobian: straight-sided simplices,
+//     parallelepipeds) use precomputed reference integrals
+//     Ghat_ba = sum_q w_q dNhat_b dNhat_a^T, so a block costs two 3x3x3
+//     products: G = Minv^T Ghat Minv |det J|.  Curved elements integrate per
+//     Gauss point from per-GP inverse Jacobians.
+//   * K_ba[d][c] = lam G[d][c] + mu G[c][d] + mu tr(G) delta_dc for Hooke's
+//     law, or sum C[d][al][c][be] G[al][be] for a general 6x6 D.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace llfem {
+
+namespace {
+
+constexpr int kMaxQ = 1024;
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double inv3(const double* J, double* Mi) {
+  // cofactor inverse, as inv3x3_fast (src/linear.jl:89-114); J row-major
+  // J[i][al] = dx_i/dxi_al; Mi row-major = J^{-1}, Mi[al][i] = dxi_al/dx_i.
+  double c11 = J[4] * J[8] - J[5] * J[7];
+  double c12 = -(J[3] * J[8] - J[5] * J[6]);
+  double c13 = J[3] * J[7] - J[4] * J[6];
+  double c21 = -(J[1] * J[8] - J[2] * J[7]);
+  double c22 = J[0] * J[8] - J[2] * J[6];
+  double c23 = -(J[0] * J[7] - J[1] * J[6]);
+  double c31 = J[1] * J[5] - J[2] * J[4];
+  double c32 = -(J[0] * J[5] - J[2] * J[3]);
+  double c33 = J[0] * J[4] - J[1] * J[3];
+  double det = J[0] * c11 + J[1] * c12 + J[2] * c13;
+  double id = 1.0 / det;
+  Mi[0] = c11 * id; Mi[1] = c21 * id; Mi[2] = c31 * id;
+  Mi[3] = c12 * id; Mi[4] = c22 * id; Mi[5] = c32 * id;
+  Mi[6] = c13 * id; Mi[7] = c23 * id; Mi[8] = c33 * id;
+  return det;
+}
+
+__device__ __forceinline__ void jacobian(int64_t E, int64_t N, int npe, int64_t e,
+                                         const double* __restrict__ xyz,
+                                         const int32_t* __restrict__ conn,
+                                         const double* __restrict__ dNq, double* J) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+  for (int a = 0; a < npe; ++a) {
+    const int32_t v = conn[(int64_t)a * E + e];
+    const double x = xyz[v], y = xyz[N + v], z = xyz[2 * N + v];
+    const double g0 = dNq[3 * a], g1 = dNq[3 * a + 1], g2 = dNq[3 * a + 2];
+    J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
+    J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
+    J[6] += z * g0; J[7] += z * g1; J[8] += z * g2;
+  }
+}
+
+// flags |= 1 if some element's Jacobian varies over the points, |= 2 if some
+// |det J| is zero or not finite (the reference's singular-geometry error).
+__global__ void k_geom_check(int64_t E, int64_t N, int npe, int q, const double* __restrict__ xyz,
+                             const int32_t* __restrict__ conn, const double* __restrict__ dN,
+                             int* __restrict__ flags) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double J0[9], J[9], Mi[9];
+  jacobian(E, N, npe, e, xyz, conn, dN, J0);
+  double scale = 0.0;
+  for (int k = 0; k < 9; ++k) scale = fmax(scale, fabs(J0[k]));
+  int f = 0;
+  double det = inv3(J0, Mi);
+  if (!(fabs(det) > 0.0) || !isfinite(det)) f |= 2;
+  for (int p = 1; p < q; ++p) {
+    jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
+    double diff = 0.0;
+    for (int k = 0; k < 9; ++k) diff = fmax(diff, fabs(J[k] - J0[k]));
+    if (diff > 1e-13 * scale) f |= 1;
+    double d = inv3(J, Mi);
+    if (!(fabs(d) > 0.0) || !isfinite(d)) f |= 2;
+  }
+  if (f) atomicOr(flags, f);
+}
+
+// Affine: geo[m*E+e] = Minv (m<9), geo[9E+e] = |det J|.
+// Per point: geo[(p*10+m)*E+e] = Minv_p, geo[(p*10+9)*E+e] = |det J_p| w_p.
+__global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
+                           const double* __restrict__ xyz, const int32_t* __restrict__ conn,
+                           const double* __restrict__ dN, const double* __restrict__ w,
+                           double* __restrict__ geo) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double J[9], Mi[9];
+  const int np = affine ? 1 : q;
+  for (int p = 0; p < np; ++p) {
+    jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
+    double det = inv3(J, Mi);
+    double* g = geo + (size_t)p * 10 * E;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) g[m * E + e] = Mi[m];
+    g[9 * E + e] = affine ? fabs(det) : fabs(det) * w[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// node-panel kernel
+// ---------------------------------------------------------------------------
+struct PanelParams {
+  const int32_t* order;
+  int64_t begin, end;
+  const int64_t* adj_ptr;
+  const int32_t* inc_ptr;
+  const int32_t* inc_elem;
+  const uint8_t* inc_loc;
+  const void* inc_map;
+  const double* geo;
+  int64_t E;
+  int q;
+  const double* tab;   // see assemble(): Ghat / Mhat / dN / N
+  double lam, mu, rho;
+  double* out;
+  int lds_per_wave;    // doubles
+  double C[81];        // general constitutive tensor C[d][al][c][be]
+};
+
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
+__global__ __launch_bounds__(256) void k_panel(PanelParams P) {
+  extern __shared__ double lds[];
+  constexpr int NPW = 64 / NPE;                 // nodes per wavefront
+  constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int64_t first = P.begin + ((int64_t)blockIdx.x * nwaves + wave) * NPW;
+  double* wl = lds + (size_t)wave * P.lds_per_wave;
+  const int n = lane / NPE;
+  const int b = lane - n * NPE;
+  const bool has = (n < NPW) && (first + n < P.end);
+  const int j = has ? P.order[first + n] : 0;
+  const int64_t a0 = has ? P.adj_ptr[j] : 0;
+  const int R = has ? (int)(P.adj_ptr[j + 1] - a0) : 0;
+  const int k0 = has ? P.inc_ptr[j] : 0;
+  const int cnt = has ? P.inc_ptr[j + 1] - k0 : 0;
+
+  int base = 0, total = 0, maxcnt = 0;
+#pragma unroll
+  for (int m = 0; m < NPW; ++m) {
+    const int Rm = __shfl(R, m * NPE);
+    const int cm = __shfl(cnt, m * NPE);
+    if (m < n) base += BLK * Rm;
+    total += BLK * Rm;
+    maxcnt = cm > maxcnt ? cm : maxcnt;
+  }
+  for (int i = lane; i < total; i += 64) wl[i] = 0.0;
+  __syncthreads();
+
+  const MapT* __restrict__ map = static_cast<const MapT*>(P.inc_map);
+  const int64_t E = P.E;
+  for (int t = 0; t < maxcnt; ++t) {
+    if (t < cnt) {
+      const int k = k0 + t;
+      const int e = P.inc_elem[k];
+      const int a = P.inc_loc[k];
+      const int s = map[(int64_t)k * NPE + b];
+      if constexpr (OP == LLFEM_OP_STIFFNESS) {
+        Real G[9];
+        Real scale;
+        if constexpr (AFFINE) {
+          Real Mi[9];
+#pragma unroll
+          for (int m = 0; m < 9; ++m) Mi[m] = (Real)P.geo[m * E + e];
+          scale = (Real)P.geo[9 * E + e];
+          const double* gh = P.tab + (size_t)(b * NPE + a) * 9;
+          Real Gh[9];
+#pragma unroll
+          for (int m = 0; m < 9; ++m) Gh[m] = (Real)gh[m];
+          Real T[9];   // T[al][k] = sum_be Gh[al][be] Mi[be][k]
+#pragma unroll
+          for (int al = 0; al < 3; ++al)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+              T[al * 3 + kk] = Gh[al * 3] * Mi[kk] + Gh[al * 3 + 1] * Mi[3 + kk] +
+                               Gh[al * 3 + 2] * Mi[6 + kk];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+              G[i * 3 + kk] = Mi[i] * T[kk] + Mi[3 + i] * T[3 + kk] + Mi[6 + i] * T[6 + kk];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 9; ++m) G[m] = 0;
+          scale = 1;
+          for (int p = 0; p < P.q; ++p) {
+            const double* g = P.geo + (size_t)p * 10 * E;
+            Real Mi[9];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) Mi[m] = (Real)g[m * E + e];
+            const Real wd = (Real)g[9 * E + e];
+            const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
+            const double* ha = P.tab + ((size_t)p * NPE + a) * 3;
+            Real gb[3], ga[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              gb[i] = Mi[i] * (Real)hb[0] + Mi[3 + i] * (Real)hb[1] + Mi[6 + i] * (Real)hb[2];
+              ga[i] = (Mi[i] * (Real)ha[0] + Mi[3 + i] * (Real)ha[1] + Mi[6 + i] * (Real)ha[2]) * wd;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) G[i * 3 + kk] += gb[i] * ga[kk];
+          }
+        }
+        Real K[9];   // K[d][c] = Ke[(b,d),(a,c)]
+        if constexpr (ISO) {
+          const Real l = (Real)P.lam * scale, u = (Real)P.mu * scale;
+          const Real tr = u * (G[0] + G[4] + G[8]);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d] + (d == c ? tr : (Real)0);
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              Real acc = 0;
+#pragma unroll
+              for (int al = 0; al < 3; ++al)
+#pragma unroll
+                for (int be = 0; be < 3; ++be)
+                  acc += (Real)P.C[((d * 3 + al) * 3 + c) * 3 + be] * G[al * 3 + be];
+              K[d * 3 + c] = acc * scale;
+            }
+        }
+        double* dst = wl + base + 3 * s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) lds_add(dst + c * 3 * R + d, (double)K[d * 3 + c]);
+      } else {
+        double mv;
+        if constexpr (AFFINE) {
+          mv = P.rho * P.geo[9 * E + e] * P.tab[b * NPE + a];
+        } else {
+          mv = 0.0;
+          for (int p = 0; p < P.q; ++p)
+            mv += P.geo[((size_t)p * 10 + 9) * E + e] * P.tab[p * NPE + b] * P.tab[p * NPE + a];
+          mv *= P.rho;
+        }
+        lds_add(wl + base + s, mv);
+      }
+    }
+  }
+  __syncthreads();
+
+  int bm = 0;
+#pragma unroll
+  for (int m = 0; m < NPW; ++m) {
+    const int Rm = __shfl(R, m * NPE);
+    const long long om = __shfl((long long)a0, m * NPE);
+    if constexpr (OP == LLFEM_OP_STIFFNESS) {
+      double* o = P.out + 9 * om;
+      for (int i = lane; i < 9 * Rm; i += 64) o[i] = wl[bm + i];
+    } else {
+      double* o = P.out + 3 * om;
+      for (int i = lane; i < 3 * Rm; i += 64) {
+        const int c = i / Rm;
+        o[i] = wl[bm + i - c * Rm];
+      }
+    }
+    bm += BLK * Rm;
+  }
+}
+
+// lumped mass: row sums of the consistent mass panels (src/linear.jl:1999-2001)
+__global__ void k_lump(int64_t N, const int64_t* __restrict__ adj_ptr, const double* __restrict__ M,
+                       double* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int64_t a0 = adj_ptr[j];
+  const int R = (int)(adj_ptr[j + 1] - a0);
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int i = 0; i < R; ++i) s += M[3 * a0 + (int64_t)c * R + i];
+    out[3 * j + c] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+using PanelFn = void (*)(PanelParams);
+
+template <int NPE, typename MapT>
+PanelFn pick(int op, bool affine, bool iso, bool fp32) {
+  if (op == LLFEM_OP_MASS)
+    return affine ? k_panel<NPE, MapT, LLFEM_OP_MASS, true, true, double>
+                  : k_panel<NPE, MapT, LLFEM_OP_MASS, false, true, double>;
+  if (affine) {
+    if (iso) return fp32 ? k_panel<NPE, MapT, 0, true, true, float> : k_panel<NPE, MapT, 0, true, true, double>;
+    return fp32 ? k_panel<NPE, MapT, 0, true, false, float> : k_panel<NPE, MapT, 0, true, false, double>;
+  }
+  if (iso) return fp32 ? k_panel<NPE, MapT, 0, false, true, float> : k_panel<NPE, MapT, 0, false, true, double>;
+  return fp32 ? k_panel<NPE, MapT, 0, false, false, float> : k_panel<NPE, MapT, 0, false, false, double>;
+}
+
+PanelFn select_panel(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32) {
+  if (map_bytes == 1) {
+    switch (npe) {
+      case 4: return pick<4, uint8_t>(op, affine, iso, fp32);
+      case 6: return pick<6, uint8_t>(op, affine, iso, fp32);
+      case 8: return pick<8, uint8_t>(op, affine, iso, fp32);
+      case 10: return pick<10, uint8_t>(op, affine, iso, fp32);
+      case 27: return pick<27, uint8_t>(op, affine, iso, fp32);
+      default: return nullptr;
+    }
+  }
+  switch (npe) {
+    case 8: return pick<8, uint16_t>(op, affine, iso, fp32);
+    case 10: return pick<10, uint16_t>(op, affine, iso, fp32);
+    case 27: return pick<27, uint16_t>(op, affine, iso, fp32);
+    case 64: return pick<64, uint16_t>(op, affine, iso, fp32);
+    default: return nullptr;
+  }
+}
+
+// Voigt index of the symmetric pair {i,j} in SymGrad row order
+// [xx,yy,zz,xy,xz,yz] (src/multifield.jl:635-657).
+int voigt(int i, int j) {
+  if (i == j) return i;
+  int a = i < j ? i : j, b = i < j ? j : i;
+  if (a == 0 && b == 1) return 3;
+  if (a == 0 && b == 2) return 4;
+  return 5;
+}
+
+int ensure_buf(llfem_ctx* ctx, double** p, size_t* cur, size_t bytes) {
+  if (*p && *cur >= bytes) return LLFEM_OK;
+  if (*p) dfree(ctx, *p, *cur);
+  *p = nullptr;
+  *cur = 0;
+  LLFEM_TRY(dmalloc(ctx, (void**)p, bytes));
+  *cur = bytes;
+  return LLFEM_OK;
+}
+
+}  // namespace
+
+// Upload [dN | w | op-table space] and run the geometry check once per rule.
+int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
+  llfem_ctx* ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  const int npe = m->npe, q = d->q;
+  uint64_t key = hash_bytes(d->dN_ref, sizeof(double) * (size_t)q * npe * 3);
+  key = hash_bytes(d->weights, sizeof(double) * (size_t)q, key);
+  RuleState& rs = m->rules[key];
+  const size_t geo_tab = (size_t)q * npe * 3 + q;
+  const size_t op_max = std::max<size_t>((size_t)npe * npe * 9, (size_t)q * npe * 3);
+  LLFEM_TRY(ensure_buf(ctx, &m->d_tab, &m->tab_bytes, sizeof(double) * (geo_tab + op_max)));
+  LLFEM_HIP(hipMemcpyAsync(m->d_tab, d->dN_ref, sizeof(double) * (size_t)q * npe * 3,
+                           hipMemcpyHostToDevice, st));
+  LLFEM_HIP(hipMemcpyAsync(m->d_tab + (size_t)q * npe * 3, d->weights, sizeof(double) * q,
+                           hipMemcpyHostToDevice, st));
+  if (!rs.checked) {
+    int* d_flags = nullptr;
+    LLFEM_TRY(dalloc(ctx, &d_flags, 1));
+    LLFEM_HIP(hipMemsetAsync(d_flags, 0, sizeof(int), st));
+    k_geom_check<<<(unsigned)((m->E + 255) / 256), 256, 0, st>>>(m->E, m->N, npe, q, m->d_xyz,
+                                                                 m->d_conn, m->d_tab, d_flags);
+    int hf = 0;
+    LLFEM_HIP(hipMemcpyAsync(&hf, d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    drelease(ctx, d_flags, 1);
+    rs.checked = true;
+    rs.affine = !(hf & 1);
+    rs.singular = (hf & 2) != 0;
+  }
+  if (rs.singular)
+    return fail(LLFEM_EGEOM, "compute_element_geometry!: singular geometry at integration point.");
+  *affine_out = rs.affine;
+  return LLFEM_OK;
+}
+
+int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
+  llfem_ctx* ctx = m->ctx;
+  const int npe = m->npe, q = d->q;
+  const size_t geo_doubles = (size_t)10 * m->E * (affine ? 1 : q);
+  LLFEM_TRY(ensure_buf(ctx, &m->d_geo, &m->geo_bytes, sizeof(double) * geo_doubles));
+  k_geometry<<<(unsigned)((m->E + 255) / 256), 256, 0, ctx->stream>>>(
+      m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,
+      m->d_tab + (size_t)q * npe * 3, m->d_geo);
+  return LLFEM_OK;
+}
+
+}  // namespace llfem
+
+using namespace llfem;
+
+extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out,
+                              llfem_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!m || !d) return fail(LLFEM_EINVAL, "llfem_assemble: NULL argument");
+  if (d->op != LLFEM_OP_STIFFNESS && d->op != LLFEM_OP_MASS)
+    return fail(LLFEM_EINVAL, "llfem_assemble: unknown operator");
+  if (d->q <= 0 || d->q > kMaxQ || !d->weights || !d->dN_ref)
+    return fail(LLFEM_EINVAL, "llfem_assemble: bad quadrature tables");
+  if (d->op == LLFEM_OP_MASS && !d->N_ref)
+    return fail(LLFEM_EINVAL, "llfem_assemble: mass needs N_ref");
+  if (d->precision == LLFEM_BF16X3)
+    return fail(LLFEM_EUNSUP, "llfem_assemble: BF16x3 is for order >= 3 hexahedra");
+  if (d->precision != LLFEM_FP64 && d->precision != LLFEM_FP32_INT_FP64_SUM)
+    return fail(LLFEM_EINVAL, "llfem_assemble: unknown precision");
+  llfem_ctx* ctx = m->ctx;
+  LLFEM_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int npe = m->npe, q = d->q, op = d->op;
+  const bool iso = (d->D == nullptr);
+  const bool fp32 = (d->precision == LLFEM_FP32_INT_FP64_SUM) && op == LLFEM_OP_STIFFNESS;
+
+  LLFEM_HIP(hipEventRecord(ctx->ev[0], st));
+  bool affine = false;
+  LLFEM_TRY(geometry_state(m, d, &affine));
+
+  PanelFn fn = select_panel(npe, m->map_bytes, op, affine, iso, fp32);
+  if (!fn)
+    return fail(LLFEM_EUNSUP, "llfem_assemble: no panel kernel for npe=" + std::to_string(npe));
+
+  // per-operator reference tables (host, FP64)
+  std::vector<double> tab;
+  if (op == LLFEM_OP_STIFFNESS) {
+    if (affine) {
+      tab.assign((size_t)npe * npe * 9, 0.0);
+      for (int b = 0; b < npe; ++b)
+        for (int a = 0; a < npe; ++a)
+          for (int al = 0; al < 3; ++al)
+            for (int be = 0; be < 3; ++be) {
+              double s = 0.0;
+              for (int p = 0; p < q; ++p)
+                s += d->weights[p] * d->dN_ref[((size_t)p * npe + b) * 3 + al] *
+                     d->dN_ref[((size_t)p * npe + a) * 3 + be];
+              tab[((size_t)b * npe + a) * 9 + al * 3 + be] = s;
+            }
+    } else {
+      tab.assign(d->dN_ref, d->dN_ref + (size_t)q * npe * 3);
+    }
+  } else {
+    if (affine) {
+      tab.assign((size_t)npe * npe, 0.0);
+      for (int b = 0; b < npe; ++b)
+        for (int a = 0; a < npe; ++a) {
+          double s = 0.0;
+          for (int p = 0; p < q; ++p)
+            s += d->weights[p] * d->N_ref[(size_t)p * npe + b] * d->N_ref[(size_t)p * npe + a];
+          tab[(size_t)b * npe + a] = s;
+        }
+    } else {
+      tab.assign(d->N_ref, d->N_ref + (size_t)q * npe);
+    }
+  }
+  const size_t off = (size_t)q * npe * 3 + q;
+  LLFEM_HIP(hipMemcpyAsync(m->d_tab + off, tab.data(), sizeof(double) * tab.size(),
+                           hipMemcpyHostToDevice, st));
+  LLFEM_TRY(run_geometry(m, d, affine));
+
+  double** outp = (op == LLFEM_OP_STIFFNESS) ? &m->d_valK : &m->d_valM;
+  const int blk = (op == LLFEM_OP_STIFFNESS) ? 9 : 3;
+  const int64_t nnz = m->nadj * blk;
+  if (!*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)nnz));
+
+  PanelParams P{};
+  P.order = m->d_order;
+  P.adj_ptr = m->d_adj_ptr;
+  P.inc_ptr = m->d_inc_ptr;
+  P.inc_elem = m->d_inc_elem;
+  P.inc_loc = m->d_inc_loc;
+  P.inc_map = m->d_inc_map;
+  P.geo = m->d_geo;
+  P.E = m->E;
+  P.q = q;
+  P.tab = m->d_tab + off;
+  P.lam = 0; P.mu = 0;
+  if (op == LLFEM_OP_STIFFNESS && iso) {
+    // elastic_constants (src/general.jl:171-175)
+    P.mu = d->E / (2.0 * (1.0 + d->nu));
+    P.lam = 2.0 * P.mu * d->nu / (1.0 - 2.0 * d->nu);
+  }
+  if (op == LLFEM_OP_STIFFNESS && !iso)
+    for (int dd = 0; dd < 3; ++dd)
+      for (int al = 0; al < 3; ++al)
+        for (int c = 0; c < 3; ++c)
+          for (int be = 0; be < 3; ++be)
+            P.C[((dd * 3 + al) * 3 + c) * 3 + be] = d->D[voigt(dd, al) * 6 + voigt(c, be)];
+  P.rho = d->rho;
+  P.out = *outp;
+  const int npw = 64 / npe;
+  const int blkl = (op == LLFEM_OP_STIFFNESS) ? 9 : 1;
+  LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
+  for (const NodeClass& c : m->classes) {
+    P.begin = c.begin;
+    P.end = c.end;
+    P.lds_per_wave = npw * c.max_r * blkl;
+    const size_t wave_bytes = sizeof(double) * (size_t)P.lds_per_wave;
+    int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, 65536 / std::max<size_t>(wave_bytes, 1)));
+    if (wave_bytes * wpb > 160 * 1024)
+      return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
+    const int64_t nodes = c.end - c.begin;
+    const int64_t waves = (nodes + npw - 1) / npw;
+    const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * wpb), wave_bytes * wpb, st, P);
+  }
+  LLFEM_HIP(hipGetLastError());
+  LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
+  if (op == LLFEM_OP_MASS && d->lumped) {
+    if (!m->d_lumped) LLFEM_TRY(dalloc(ctx, &m->d_lumped, 3 * m->N));
+    k_lump<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_valM, m->d_lumped);
+  }
+  LLFEM_HIP(hipEventRecord(ctx->ev[3], st));
+  if (op == LLFEM_OP_STIFFNESS) {
+    m->have_K = true;
+    m->solver_ready = false;
+  }
+  if (nzval_out) {
+    if (op == LLFEM_OP_MASS && d->lumped)
+      LLFEM_HIP(hipMemcpyAsync(nzval_out, m->d_lumped, sizeof(double) * 3 * m->N,
+                               hipMemcpyDeviceToHost, st));
+    else
+      LLFEM_HIP(hipMemcpyAsync(nzval_out, *outp, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
+  }
+  LLFEM_HIP(hipEventSynchronize(ctx->ev[3]));
+  if (nzval_out) LLFEM_HIP(hipStreamSynchronize(st));
+  if (stats) {
+    float ms = 0.f, ms_main = 0.f;
+    LLFEM_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    LLFEM_HIP(hipEventElapsedTime(&ms_main, ctx->ev[1], ctx->ev[2]));
+    stats->kernel_ms = ms;
+    stats->main_kernel_ms = ms_main;
+    stats->nnz = nnz;
+    stats->elements = m->E;
+    stats->alg_bytes = 24.0 * m->N + 4.0 * npe * m->E + 8.0 * nnz;
+    stats->iterations = 0;
+    stats->refinements = 0;
+    stats->relres = 0;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return LLFEM_OK;
+}

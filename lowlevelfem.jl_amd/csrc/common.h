@@ -1,0 +1,136 @@
+// Internal definitions shared by the libllfem translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "llfem.h"
+
+namespace llfem {
+
+void set_error(const std::string& msg);
+
+#define LLFEM_HIP(expr)                                                            \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      ::llfem::set_error(std::string("HIP error: ") + hipGetErrorString(_e) +      \
+                         " at " + __FILE__ + ":" + std::to_string(__LINE__) +     \
+                         " (" #expr ")");                                          \
+      return LLFEM_EHIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+#define LLFEM_TRY(expr)          \
+  do {                           \
+    int _s = (expr);             \
+    if (_s != LLFEM_OK) return _s; \
+  } while (0)
+
+inline int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+// Nodes that share an incidence count form one launch class (no divergence
+// in the incidence loop); max_r sizes the LDS panels of the class.
+struct NodeClass {
+  int64_t begin, end;   // range in the processing order
+  int32_t count;        // incidences per node
+  int32_t max_r;        // largest node-adjacency row count in the class
+};
+
+// Per-rule geometry state: affine detection is a property of the mesh and
+// the rule's points, computed once.
+struct RuleState {
+  bool checked = false;
+  bool affine = false;
+  bool singular = false;
+};
+
+}  // namespace llfem
+
+struct llfem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int num_cus = 0;
+};
+
+struct llfem_mesh {
+  llfem_ctx* ctx = nullptr;
+  int64_t N = 0, E = 0;
+  int32_t npe = 0, etype = 0, pdim = 3;
+  // coordinates SoA x[N] y[N] z[N]; connectivity SoA [npe][E]
+  double* d_xyz = nullptr;
+  int32_t* d_conn = nullptr;
+  // node adjacency (sorted) = the structural pattern at node level
+  int64_t nadj = 0;
+  int64_t* d_adj_ptr = nullptr;   // [N+1]
+  int32_t* d_adj = nullptr;       // [nadj]
+  int32_t* d_diag_slot = nullptr; // [N] position of j in its own row list
+  // incidences sorted by (node, element)
+  int64_t ninc = 0;
+  int32_t* d_inc_ptr = nullptr;   // [N+1]
+  int32_t* d_inc_elem = nullptr;  // [ninc]
+  uint8_t* d_inc_loc = nullptr;   // [ninc]
+  void* d_inc_map = nullptr;      // [ninc][npe] slot of each element node in the panel
+  int32_t map_bytes = 1;
+  int32_t* d_order = nullptr;     // [N] nodes sorted by incidence count
+  std::vector<llfem::NodeClass> classes;
+  int32_t max_r = 0;
+  // host copies kept for pattern export
+  std::vector<int64_t> h_adj_ptr;
+  // per-assembly scratch
+  double* d_geo = nullptr;
+  size_t geo_bytes = 0;
+  double* d_tab = nullptr;
+  size_t tab_bytes = 0;
+  std::map<uint64_t, llfem::RuleState> rules;
+  // device-resident results
+  double* d_valK = nullptr;
+  double* d_valM = nullptr;
+  double* d_lumped = nullptr;
+  bool have_K = false;
+  // solver state cached per assembled K
+  float* d_valK32 = nullptr;
+  double* d_diagK = nullptr;
+  bool solver_ready = false;
+};
+
+namespace llfem {
+
+int dmalloc(llfem_ctx* ctx, void** p, size_t bytes);
+void dfree(llfem_ctx* ctx, void* p, size_t bytes);
+
+template <typename T>
+int dalloc(llfem_ctx* ctx, T** p, size_t count) {
+  return dmalloc(ctx, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+template <typename T>
+void drelease(llfem_ctx* ctx, T*& p, size_t count) {
+  if (p) dfree(ctx, p, count * sizeof(T));
+  p = nullptr;
+}
+
+int build_pattern(llfem_mesh* m);
+
+// Element-type facts the kernels need.
+inline int etype_dim(int et) {
+  switch (et) {
+    case 4: case 5: case 6: case 11: case 12: case 92: case 93: return 3;
+    case 2: case 3: case 9: case 10: case 36: case 37: return 2;
+    default: return 0;
+  }
+}
+
+uint64_t hash_bytes(const void* p, size_t n, uint64_t h = 1469598103934665603ull);
+
+}  // namespace llfem

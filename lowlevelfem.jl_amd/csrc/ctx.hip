@@ -1,0 +1,100 @@
+// Context, error reporting and device-memory accounting.
+#include "common.h"
+
+#include <cstring>
+
+namespace llfem {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int dmalloc(llfem_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return LLFEM_OK;
+  if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+    return fail(LLFEM_ENOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
+                                  " more bytes, " + std::to_string(ctx->in_use) + " of " +
+                                  std::to_string(ctx->budget) + " in use");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(LLFEM_ENOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                  ") failed: " + hipGetErrorString(e));
+  }
+  ctx->in_use += bytes;
+  return LLFEM_OK;
+}
+
+void dfree(llfem_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  ctx->in_use -= bytes;
+}
+
+uint64_t hash_bytes(const void* p, size_t n, uint64_t h) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+}  // namespace llfem
+
+using namespace llfem;
+
+extern "C" {
+
+const char* llfem_last_error(void) { return g_error.c_str(); }
+
+int llfem_ctx_create(int device, size_t hbm_budget_bytes, llfem_ctx** out) {
+  if (!out) return fail(LLFEM_EINVAL, "llfem_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  LLFEM_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fail(LLFEM_EINVAL, "llfem_ctx_create: device " + std::to_string(device) +
+                                  " out of range (" + std::to_string(ndev) + " devices)");
+  LLFEM_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  LLFEM_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(LLFEM_EUNSUP, std::string("libllfem is built for gfx950; device is ") +
+                                  prop.gcnArchName);
+  llfem_ctx* c = new llfem_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  c->num_cus = prop.multiProcessorCount;
+  // default-priority stream (never a priority stream)
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(LLFEM_EHIP, "hipStreamCreate failed");
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return LLFEM_OK;
+}
+
+int llfem_ctx_destroy(llfem_ctx* ctx) {
+  if (!ctx) return LLFEM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LLFEM_OK;
+}
+
+int llfem_ctx_synchronize(llfem_ctx* ctx) {
+  if (!ctx) return fail(LLFEM_EINVAL, "llfem_ctx_synchronize: NULL context");
+  LLFEM_HIP(hipStreamSynchronize(ctx->stream));
+  return LLFEM_OK;
+}
+
+size_t llfem_ctx_bytes_in_use(const llfem_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+}  // extern "C"

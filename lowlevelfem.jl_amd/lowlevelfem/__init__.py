@@ -1,0 +1,384 @@
+"""Host-side mirror of LowLevelFEM.jl's operator-assembly API, backed by the
+MI355X library libllfem.so.
+
+Same names, argument meaning and error behaviour as the reference for the
+hot path (LowLevelFEM.jl 1.13.31):
+
+    structured_box_mesh  src/general.jl:8910-8996
+    Material             src/general.jl:99-140, elastic_constants :142-203
+    Problem              src/general.jl:478-632 (type=:Solid)
+    SystemMatrix         src/general.jl:751-780
+    VectorField          src/general.jl:1554-1563 (container only)
+    stiffnessMatrix      src/linear.jl:70-87 -> stiffnessMatrixSolid :190-330
+    massMatrix           src/linear.jl:1843-2004
+    loadVector           src/linear.jl:3088-3434 (traction loads)
+    solveDisplacement    src/linear.jl:3711-3743
+    displacementConstraint / load / BoundaryCondition / LoadCondition
+                         src/general.jl:2480-2605, 4179-4233
+    constrainedDoFs / freeDoFs / allDoFs / nodesOnPhysicalGroup
+                         src/general.jl:5249-5492
+    build_csc_pattern    src/multifield.jl:2083-2250
+    SymGrad / integral   the ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); ...) DSL path,
+                         src/multifield.jl:5295-5339, assemble_operator :2630-3280
+
+Differences that follow from the host language: DoF and node indices are
+0-based (dof = 3*node + c); sparse matrices are scipy.sparse.csc_matrix with
+int64 indices; the mesh is an explicit object (the reference keeps it in
+gmsh's global state -- structured_box_mesh also sets the current mesh here).
+There is no CPU fallback: every operator runs through libllfem.so.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from . import refelem
+from .mesh import Mesh, structured_box_mesh as _sbm, structural_nnz
+
+__all__ = [
+    "Material", "Problem", "SystemMatrix", "VectorField", "BoundaryCondition", "LoadCondition",
+    "structured_box_mesh", "stiffnessMatrix", "massMatrix", "loadVector", "solveDisplacement",
+    "displacementConstraint", "load", "constrainedDoFs", "freeDoFs", "allDoFs",
+    "nodesOnPhysicalGroup", "build_csc_pattern", "SymGrad", "integral", "dropzeros",
+    "elastic_constants", "Mesh",
+]
+
+_current_mesh: Mesh | None = None
+
+
+def structured_box_mesh(**kw) -> Mesh:
+    """Build a structured box mesh and make it the current model (the
+    reference's gmsh.model.add("structured_box"), src/general.jl:8910)."""
+    global _current_mesh
+    _current_mesh = _sbm(**kw)
+    return _current_mesh
+
+
+def elastic_constants(E=None, nu=None, lam=None, mu=None, kappa=None):
+    """Restates elastic_constants (src/general.jl:142-203)."""
+    given = {k: v for k, v in dict(E=E, nu=nu, lam=lam, mu=mu, kappa=kappa).items() if v is not None}
+    n = len(given)
+    if n == 0 or (n == 1 and "E" in given):
+        E = given.get("E", 2.0e5)
+        nu = 0.3
+        mu = E / (2 * (1 + nu)); lam = 2 * mu * nu / (1 - 2 * nu); kappa = E / (3 * (1 - 2 * nu))
+    elif n == 1 and "kappa" in given:
+        kappa = given["kappa"]; E = 2.0e5; nu = (1 - E / 3 / kappa) / 2
+        mu = E / (2 * (1 + nu)); lam = 2 * mu * nu / (1 - 2 * nu)
+    elif n == 2 and "E" in given and "nu" in given:
+        E, nu = given["E"], given["nu"]
+        mu = E / (2 * (1 + nu)); lam = 2 * mu * nu / (1 - 2 * nu); kappa = E / (3 * (1 - 2 * nu))
+    elif n == 2 and "mu" in given and "nu" in given:
+        mu, nu = given["mu"], given["nu"]
+        E = 2 * mu * (1 + nu); lam = 2 * mu * nu / (1 - 2 * nu); kappa = E / (3 * (1 - 2 * nu))
+    elif n == 2 and "lam" in given and "mu" in given:
+        lam, mu = given["lam"], given["mu"]
+        nu = lam / (2 * (lam + mu)); E = mu * (3 * lam + 2 * mu) / (lam + mu); kappa = lam + 2 * mu / 3
+    elif n == 2 and "kappa" in given and "mu" in given:
+        kappa, mu = given["kappa"], given["mu"]
+        lam = kappa - 2 * mu / 3; nu = lam / (2 * (lam + mu)); E = 2 * mu * (1 + nu)
+    elif n < 2:
+        raise ValueError("Insufficient elastic parameters. Specify at least E+ν, μ+ν, λ+μ, κ+μ, or E alone.")
+    else:
+        raise ValueError(f"Elastic constants are overdetermined: {tuple(given)}")
+    return dict(E=E, nu=nu, lam=lam, mu=mu, kappa=kappa)
+
+
+@dataclass
+class Material:
+    phName: str = "body"
+    type: str = "Hooke"
+    E: float | None = None
+    nu: float | None = None
+    lam: float | None = None
+    mu: float | None = None
+    kappa: float | None = None
+    rho: float = 7.85e-9
+
+    def __post_init__(self):
+        if self.type not in ("Hooke", "StVenantKirchhoff", "NeoHookeCompressible", "JFO"):
+            raise ValueError(f"Invalid material type: {self.type}")
+        ec = elastic_constants(self.E, self.nu, self.lam, self.mu, self.kappa)
+        self.E, self.nu, self.lam, self.mu, self.kappa = (ec["E"], ec["nu"], ec["lam"], ec["mu"],
+                                                          ec["kappa"])
+
+
+class Problem:
+    """A :Solid problem on a mesh (src/general.jl:478-632).  ``non`` is the
+    node count; dim = pdim = 3."""
+
+    def __init__(self, materials, type="Solid", mesh: Mesh | None = None, context=None):
+        if type not in ("Solid", "dummy"):
+            raise ValueError("only :Solid (3D linear elasticity) problems are on the GPU path")
+        self.material = list(materials)
+        self.type = type
+        self.mesh = mesh if mesh is not None else _current_mesh
+        if self.mesh is None and type != "dummy":
+            raise ValueError("Problem: no mesh (call structured_box_mesh first or pass mesh=)")
+        self.dim = 3
+        self.pdim = 3
+        self.field = "u"
+        self.name = self.mesh.name if self.mesh is not None else "dummy"
+        self.non = self.mesh.N if self.mesh is not None else 0
+        self._ctx = context
+        self._dev = None
+
+    def ndofs(self):
+        return self.pdim * self.non
+
+    def device(self) -> _lib.DeviceMesh:
+        """The device-resident mesh/pattern (built once, reused)."""
+        if self._dev is None:
+            ctx = self._ctx or _lib.default_context()
+            self._dev = _lib.DeviceMesh(ctx, self.mesh.coords, self.mesh.etype, self.mesh.conn)
+        return self._dev
+
+
+@dataclass
+class SystemMatrix:
+    A: sp.csc_matrix
+    model: Problem
+    stats: dict = field(default_factory=dict)
+
+
+@dataclass
+class VectorField:
+    a: np.ndarray                  # (ndof, nsteps)
+    model: Problem
+    type: str = "v3D"
+    t: np.ndarray = field(default_factory=lambda: np.array([0.0]))
+    nsteps: int = 1
+    stats: dict = field(default_factory=dict)
+
+    def copy(self):
+        return VectorField(self.a.copy(), self.model, self.type, self.t.copy(), self.nsteps)
+
+
+@dataclass
+class BoundaryCondition:
+    phName: str
+    values: dict
+    problem: Problem | None = None
+
+
+@dataclass
+class LoadCondition:
+    phName: str
+    values: dict
+    problem: Problem | None = None
+
+
+def displacementConstraint(name, **kw):
+    """src/general.jl:4179-4183; keys ux, uy, uz."""
+    return BoundaryCondition(name, {k: v for k, v in kw.items() if v is not None})
+
+
+def load(name, **kw):
+    """src/general.jl:4207-4233; traction keys fx, fy, fz."""
+    return LoadCondition(name, {k: v for k, v in kw.items() if v is not None})
+
+
+def nodesOnPhysicalGroup(problem: Problem, phName: str) -> np.ndarray:
+    return problem.mesh.nodes_on(phName)
+
+
+_COMP = {"x": 0, "y": 1, "z": 2}
+
+
+def constrainedDoFs(problem: Problem, supports) -> np.ndarray:
+    """unique(3*node + c) over the supports (src/general.jl:5289-5322)."""
+    dofs = []
+    for bc in supports:
+        if bc.problem is not None and bc.problem is not problem:
+            continue
+        nodes = nodesOnPhysicalGroup(problem, bc.phName)
+        for key in bc.values:
+            if not key.startswith(problem.field):
+                continue
+            c = _COMP[key[len(problem.field):]]
+            dofs.append(3 * nodes + c)
+    if not dofs:
+        return np.zeros(0, dtype=np.int64)
+    d = np.concatenate(dofs)
+    _, first = np.unique(d, return_index=True)
+    return d[np.sort(first)].astype(np.int64)
+
+
+def allDoFs(problem: Problem) -> np.ndarray:
+    """All x DoFs, then y, then z (src/general.jl:5457-5472)."""
+    nodes = np.arange(problem.non, dtype=np.int64)
+    return np.concatenate([3 * nodes + c for c in range(3)])
+
+
+def freeDoFs(problem: Problem, supports) -> np.ndarray:
+    """setdiff(allDoFs, constrainedDoFs) keeping allDoFs order (src/general.jl:5487-5492)."""
+    a = allDoFs(problem)
+    return a[~np.isin(a, constrainedDoFs(problem, supports))]
+
+
+def dropzeros(A: sp.csc_matrix) -> sp.csc_matrix:
+    """SparseArrays.dropzeros!: remove stored entries that are exactly 0.0."""
+    B = A.copy()
+    B.eliminate_zeros()
+    B.indptr = B.indptr.astype(np.int64)
+    B.indices = B.indices.astype(np.int64)
+    return B
+
+
+def _csc(n, colptr, rowval, val):
+    A = sp.csc_matrix((val, rowval, colptr), shape=(n, n))
+    A.indptr = np.asarray(colptr, dtype=np.int64)
+    A.indices = np.asarray(rowval, dtype=np.int64)
+    return A
+
+
+_PREC = {"fp64": _lib.FP64, "fp32": _lib.FP32_INT_FP64_SUM, "bf16x3": _lib.BF16X3}
+
+
+def _rule(problem: Problem, gauss=None):
+    name, dim, order, npe, family = refelem.element_properties(problem.mesh.etype)
+    return refelem.rule_tables(problem.mesh.etype, refelem.gauss_degree(order, gauss))
+
+
+def build_csc_pattern(Pu: Problem, Ps: Problem | None = None) -> sp.csc_matrix:
+    """Structural CSC pattern with zero values (src/multifield.jl:2083-2250)."""
+    if Ps is not None and Ps.mesh is not Pu.mesh:
+        raise ValueError("Both problems must refer to the same gmsh model/mesh.")
+    dev = Pu.device()
+    colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
+    return _csc(Pu.ndofs(), colptr, rowval, np.zeros(rowval.size))
+
+
+def stiffnessMatrix(problem: Problem, elements=(), forceOneThread=True, gauss=None,
+                    precision="fp64", dropzeros_=True) -> SystemMatrix | None:
+    """stiffnessMatrix(problem) (src/linear.jl:70-87): Hooke material of the
+    first material group, rule "Gauss"*(2*order+1), CSC with dropzeros!
+    applied (src/linear.jl:323-324)."""
+    if problem.type == "dummy":
+        return None
+    if len(problem.material) != 1:
+        raise NotImplementedError("one material group per problem on the GPU path")
+    m = problem.material[0]
+    dev = problem.device()
+    tab = _rule(problem, gauss)
+    val, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision])
+    colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
+    A = _csc(problem.ndofs(), colptr, rowval, val)
+    if dropzeros_:
+        A = dropzeros(A)
+    return SystemMatrix(A, problem, st.as_dict())
+
+
+class SymGrad:
+    """SymGrad(P): symmetric gradient operator applied to a vector field
+    (src/multifield.jl:4460)."""
+
+    def __init__(self, P: Problem):
+        self.P = P
+
+
+def integral(a: SymGrad, D, b: SymGrad, gauss=None, csc_matrix=None, precision="fp64"):
+    """∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); gauss, csc_matrix) (src/multifield.jl:5295-5339).
+    D is the 6x6 coefficient in SymGrad row order [xx,yy,zz,xy,xz,yz].
+    Without csc_matrix the result gets dropzeros! (:5336); with it the
+    structural pattern is kept and the caller's matrix is filled in place."""
+    if not isinstance(a, SymGrad) or not isinstance(b, SymGrad) or a.P is not b.P:
+        raise NotImplementedError("only ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu)) is on the GPU path")
+    P = a.P
+    D = np.asarray(D, dtype=np.float64).reshape(6, 6)
+    dev = P.device()
+    tab = _rule(P, gauss)
+    val, st = dev.assemble(_lib.OP_STIFFNESS, tab, D=D, precision=_PREC[precision])
+    if csc_matrix is not None:
+        K = csc_matrix.A if isinstance(csc_matrix, SystemMatrix) else csc_matrix
+        if K.shape != (P.ndofs(), P.ndofs()) or K.nnz != val.size:
+            raise ValueError("assemble_operator: csc_matrix has an incompatible size/pattern")
+        K.data[:] = val
+        return SystemMatrix(K, P, st.as_dict())
+    colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
+    return SystemMatrix(dropzeros(_csc(P.ndofs(), colptr, rowval, val)), P, st.as_dict())
+
+
+def massMatrix(problem: Problem, lumped=True, gauss=None) -> SystemMatrix | None:
+    """massMatrix(problem; lumped=true) (src/linear.jl:1843-2004)."""
+    if problem.type == "dummy":
+        return None
+    m = problem.material[0]
+    dev = problem.device()
+    tab = _rule(problem, gauss)
+    val, st = dev.assemble(_lib.OP_MASS, tab, rho=m.rho, lumped=lumped)
+    n = problem.ndofs()
+    if lumped:
+        A = sp.diags(val, 0, shape=(n, n), format="csc")
+        return SystemMatrix(dropzeros(A), problem, st.as_dict())
+    colptr, rowval = dev.pattern(_lib.OP_MASS)
+    return SystemMatrix(dropzeros(_csc(n, colptr, rowval, val)), problem, st.as_dict())
+
+
+def loadVector(problem: Problem, loads) -> VectorField | None:
+    """Traction loads load(name; fx, fy, fz) with constant components
+    (src/linear.jl:3088-3434)."""
+    if problem.type == "dummy":
+        return None
+    if not isinstance(loads, (list, tuple)):
+        raise TypeError("loadVector: loads must be in a vector.")
+    dev = problem.device()
+    f = np.zeros(problem.ndofs())
+    for ld in loads:
+        if ld.problem is not None and ld.problem is not problem:
+            continue
+        t = np.zeros(3)
+        for k, v in ld.values.items():
+            if k not in ("fx", "fy", "fz"):
+                raise NotImplementedError(f"loadVector: load key {k} is not on the GPU path")
+            if callable(v):
+                raise NotImplementedError("loadVector: function-valued loads are not on the GPU path")
+            t["xyz".index(k[1])] = float(v)
+        if ld.phName not in problem.mesh.faces:
+            raise KeyError(f"Physical group '{ld.phName}' not found.")
+        fs = problem.mesh.faces[ld.phName]
+        name, dim, order, npe, family = refelem.element_properties(fs.etype)
+        tab = refelem.rule_tables(fs.etype, 2 * order + 1)
+        f += dev.traction(fs.etype, fs.conn, tab, t)
+    return VectorField(f.reshape(-1, 1), problem)
+
+
+def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=False,
+                      reltol=math.sqrt(np.finfo(float).eps), maxiter=None, preconditioner="jacobi",
+                      ordering=True, method="fp64") -> VectorField:
+    """solveDisplacement(K, f; support, ...) (src/linear.jl:3711-3743).
+
+    The GPU path is Jacobi-PCG (method "fp64", or "mixed" = FP32 inner CG
+    with FP64 refinement).  iterative=false (the reference's direct solve)
+    runs the same PCG to reltol = 1e-12; iterative=true honours reltol."""
+    problem = K.model
+    if f.type != "v3D":
+        raise ValueError(f"solveDisplacement: wrong type of 'f': ({f.type})")
+    support = list(support)
+    fixed = constrainedDoFs(problem, support)
+    uval = np.zeros(problem.ndofs())
+    for bc in support:     # applyBoundaryConditions! (src/linear.jl:3505-3563)
+        if bc.problem is not None and bc.problem is not problem:
+            continue
+        nodes = nodesOnPhysicalGroup(problem, bc.phName)
+        for key, v in bc.values.items():
+            if callable(v):
+                raise NotImplementedError("function-valued supports are not on the GPU path")
+            uval[3 * nodes + _COMP[key[1:]]] = float(v)
+    ufix = uval[fixed]
+    tol = reltol if iterative else 1e-12
+    maxiter = maxiter or max(problem.non * problem.dim, 1000)
+    meth = _lib.PCG_MIXED_REFINE if method == "mixed" else _lib.PCG_JACOBI_FP64
+    dev = problem.device()
+    A = K.A
+    fa = np.ascontiguousarray(f.a[:, 0], dtype=np.float64)
+    u, st = _lib.solve_csc(dev.ctx, A.indptr, A.indices, A.data, fa, fixed, ufix, method=meth,
+                           reltol=tol, maxiter=maxiter)
+    out = f.copy()
+    out.a[:, 0] = u
+    out.stats = st.as_dict()
+    return out

@@ -1,0 +1,222 @@
+"""ctypes binding of libllfem.so (the C ABI declared in include/llfem.h).
+
+The product path has no fallback: if the HIP library is missing or no gfx950
+device is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libllfem.so")
+
+OK, EINVAL, EHIP, ENOMEM, EGEOM, EPATTERN, ENOCONV, EUNSUP = 0, -1, -2, -3, -4, -5, -6, -7
+OP_STIFFNESS, OP_MASS = 0, 1
+FP64, FP32_INT_FP64_SUM, BF16X3 = 0, 1, 2
+PCG_JACOBI_FP64, PCG_MIXED_REFINE = 0, 1
+
+EXPORTED = [
+    "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
+    "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
+    "llfem_assemble", "llfem_load_traction", "llfem_solve", "llfem_solve_csc",
+]
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("op", C.c_int32), ("q", C.c_int32), ("weights", C.c_void_p),
+                ("N_ref", C.c_void_p), ("dN_ref", C.c_void_p), ("D", C.c_void_p),
+                ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
+                ("lumped", C.c_int32), ("precision", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
+                ("total_ms", C.c_double), ("nnz", C.c_int64), ("elements", C.c_int64),
+                ("alg_bytes", C.c_double), ("iterations", C.c_int64),
+                ("refinements", C.c_int64), ("relres", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LLFEMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libllfem.so and declare its prototypes (no device access)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"libllfem.so not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P, I32, I64, D, SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_size_t
+    lib.llfem_ctx_create.argtypes = [C.c_int, SZ, C.POINTER(P)]
+    lib.llfem_ctx_destroy.argtypes = [P]
+    lib.llfem_ctx_synchronize.argtypes = [P]
+    lib.llfem_ctx_bytes_in_use.argtypes = [P]
+    lib.llfem_ctx_bytes_in_use.restype = SZ
+    lib.llfem_last_error.restype = C.c_char_p
+    lib.llfem_mesh_create.argtypes = [P, P, I64, I32, I32, P, I64, I32, I32, C.POINTER(P)]
+    lib.llfem_mesh_destroy.argtypes = [P]
+    lib.llfem_mesh_pattern.argtypes = [P, I32, I32, C.POINTER(I64), P, P]
+    lib.llfem_assemble.argtypes = [P, C.POINTER(OpDesc), P, C.POINTER(Stats)]
+    lib.llfem_load_traction.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, P]
+    lib.llfem_solve.argtypes = [P, P, P, I64, I32, P, I32, D, I64, P, C.POINTER(Stats)]
+    lib.llfem_solve_csc.argtypes = [P, I64, P, P, P, I32, P, P, I64, P, I32, D, I64, P,
+                                    C.POINTER(Stats)]
+    for name in EXPORTED:
+        fn = getattr(lib, name)
+        if name not in ("llfem_last_error", "llfem_ctx_bytes_in_use"):
+            fn.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def check(code: int):
+    if code != OK:
+        msg = _lib.llfem_last_error().decode(errors="replace")
+        raise LLFEMError(code, msg)
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP stream + memory accounting on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        check(lib.llfem_ctx_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+        self.lib = lib
+
+    def synchronize(self):
+        check(self.lib.llfem_ctx_synchronize(self.h))
+
+    def bytes_in_use(self) -> int:
+        return int(self.lib.llfem_ctx_bytes_in_use(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.llfem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("LLFEM_DEVICE", "0")))
+    return _default_ctx
+
+
+class DeviceMesh:
+    """Device-resident mesh + structural pattern (llfem_mesh)."""
+
+    def __init__(self, ctx: Context, coords, etype, conn):
+        self.ctx = ctx
+        lib = ctx.lib
+        xyz = np.ascontiguousarray(coords, dtype=np.float64)
+        cn = np.ascontiguousarray(conn, dtype=np.int64)
+        h = C.c_void_p()
+        check(lib.llfem_mesh_create(ctx.h, ptr(xyz), xyz.shape[0], int(etype), cn.shape[1], ptr(cn),
+                                    cn.shape[0], 3, 0, C.byref(h)))
+        self.h = h
+        self.N = xyz.shape[0]
+
+    def pattern(self, op=OP_STIFFNESS, with_arrays=True):
+        lib = self.ctx.lib
+        nnz = C.c_int64()
+        check(lib.llfem_mesh_pattern(self.h, op, 0, C.byref(nnz), None, None))
+        if not with_arrays:
+            return int(nnz.value)
+        colptr = np.empty(3 * self.N + 1, dtype=np.int64)
+        rowval = np.empty(int(nnz.value), dtype=np.int64)
+        check(lib.llfem_mesh_pattern(self.h, op, 0, C.byref(nnz), ptr(colptr), ptr(rowval)))
+        return colptr, rowval
+
+    def assemble(self, op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False,
+                 precision=FP64, copy_out=True):
+        w = np.ascontiguousarray(tables.weights, dtype=np.float64)
+        Nr = np.ascontiguousarray(tables.N, dtype=np.float64)
+        dN = np.ascontiguousarray(tables.dN, dtype=np.float64)
+        Dm = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(6, 6)
+        desc = OpDesc(op, tables.q, ptr(w).value, ptr(Nr).value, ptr(dN).value,
+                      None if Dm is None else ptr(Dm).value, E, nu, rho, int(lumped), precision)
+        st = Stats()
+        out = None
+        if copy_out:
+            n = 3 * self.N if (op == OP_MASS and lumped) else self.pattern(op, with_arrays=False)
+            out = np.empty(n, dtype=np.float64)
+        check(self.ctx.lib.llfem_assemble(self.h, C.byref(desc), ptr(out), C.byref(st)))
+        return out, st
+
+    def traction(self, face_type, face_conn, tables, t):
+        fc = np.ascontiguousarray(face_conn, dtype=np.int64)
+        f = np.empty(3 * self.N, dtype=np.float64)
+        tt = np.ascontiguousarray(t, dtype=np.float64)
+        w = np.ascontiguousarray(tables.weights)
+        Nr = np.ascontiguousarray(tables.N)
+        dN = np.ascontiguousarray(tables.dN)
+        check(self.ctx.lib.llfem_load_traction(self.h, int(face_type), fc.shape[1], ptr(fc),
+                                               fc.shape[0], 0, tables.q, ptr(w), ptr(Nr), ptr(dN),
+                                               ptr(tt), ptr(f)))
+        return f
+
+    def solve(self, f, fixed, ufix=None, method=PCG_JACOBI_FP64, reltol=1e-10, maxiter=100000):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        fx = np.ascontiguousarray(fixed, dtype=np.int64)
+        uf = None if ufix is None else np.ascontiguousarray(ufix, dtype=np.float64)
+        u = np.empty_like(f)
+        st = Stats()
+        code = self.ctx.lib.llfem_solve(self.h, ptr(f), ptr(fx), fx.size, 0, ptr(uf), method,
+                                        float(reltol), int(maxiter), ptr(u), C.byref(st))
+        check(code)
+        return u, st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.llfem_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_csc(ctx: Context, colptr, rowval, nzval, f, fixed, ufix=None, method=PCG_JACOBI_FP64,
+              reltol=1e-10, maxiter=100000):
+    cp = np.ascontiguousarray(colptr, dtype=np.int64)
+    rv = np.ascontiguousarray(rowval, dtype=np.int64)
+    nz = np.ascontiguousarray(nzval, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    fx = np.ascontiguousarray(fixed, dtype=np.int64)
+    uf = None if ufix is None else np.ascontiguousarray(ufix, dtype=np.float64)
+    u = np.empty_like(f)
+    st = Stats()
+    check(ctx.lib.llfem_solve_csc(ctx.h, cp.size - 1, ptr(cp), ptr(rv), ptr(nz), 0, ptr(f), ptr(fx),
+                                  fx.size, ptr(uf), method, float(reltol), int(maxiter), ptr(u),
+                                  C.byref(st)))
+    return u, st
