@@ -61,15 +61,20 @@ __device__ __forceinline__ double inv3(const double* J, double* Mi) {
   return det;
 }
 
+// J[i][al] = sum_a x_i^a dN_a[al], evaluated on coordinates relative to the
+// element's first node (sum_a dN_a = 0 makes this the same Jacobian, without
+// the cancellation of large absolute coordinates).
 __device__ __forceinline__ void jacobian(int64_t E, int64_t N, int npe, int64_t e,
                                          const double* __restrict__ xyz,
                                          const int32_t* __restrict__ conn,
                                          const double* __restrict__ dNq, double* J) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) J[k] = 0.0;
-  for (int a = 0; a < npe; ++a) {
+  const int32_t v0 = conn[e];
+  const double x0 = xyz[v0], y0 = xyz[N + v0], z0 = xyz[2 * N + v0];
+  for (int a = 1; a < npe; ++a) {
     const int32_t v = conn[(int64_t)a * E + e];
-    const double x = xyz[v], y = xyz[N + v], z = xyz[2 * N + v];
+    const double x = xyz[v] - x0, y = xyz[N + v] - y0, z = xyz[2 * N + v] - z0;
     const double g0 = dNq[3 * a], g1 = dNq[3 * a + 1], g2 = dNq[3 * a + 2];
     J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
     J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
@@ -95,15 +100,21 @@ __global__ void k_geom_check(int64_t E, int64_t N, int npe, int q, const double*
     jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
     double diff = 0.0;
     for (int k = 0; k < 9; ++k) diff = fmax(diff, fabs(J[k] - J0[k]));
-    if (diff > 1e-13 * scale) f |= 1;
+    if (diff > 1e-12 * scale) f |= 1;
     double d = inv3(J, Mi);
     if (!(fabs(d) > 0.0) || !isfinite(d)) f |= 2;
   }
   if (f) atomicOr(flags, f);
 }
 
-// Affine: geo[m*E+e] = Minv (m<9), geo[9E+e] = |det J|.
-// Per point: geo[(p*10+m)*E+e] = Minv_p, geo[(p*10+9)*E+e] = |det J_p| w_p.
+// Geometry records, one 128-byte line each (element-major, so a gather by
+// element id touches one cache line):
+//   double[0..8] Minv (row-major J^{-1}), double[9] wd,
+//   float[20..29] the same ten values in FP32 (for the FP32 integration path).
+// Affine: one record per element, wd = |det J|.  Otherwise one record per
+// (element, point), wd = |det J_p| w_p.
+constexpr int kGeoStride = 16;   // doubles
+
 __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
                            const double* __restrict__ xyz, const int32_t* __restrict__ conn,
                            const double* __restrict__ dN, const double* __restrict__ w,
@@ -114,11 +125,19 @@ __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
   const int np = affine ? 1 : q;
   for (int p = 0; p < np; ++p) {
     jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
-    double det = inv3(J, Mi);
-    double* g = geo + (size_t)p * 10 * E;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) g[m * E + e] = Mi[m];
-    g[9 * E + e] = affine ? fabs(det) : fabs(det) * w[p];
+    const double det = inv3(J, Mi);
+    const double wd = affine ? fabs(det) : fabs(det) * w[p];
+    double* g = geo + ((size_t)e * np + p) * kGeoStride;
+    double2* g2 = reinterpret_cast<double2*>(g);
+    g2[0] = make_double2(Mi[0], Mi[1]);
+    g2[1] = make_double2(Mi[2], Mi[3]);
+    g2[2] = make_double2(Mi[4], Mi[5]);
+    g2[3] = make_double2(Mi[6], Mi[7]);
+    g2[4] = make_double2(Mi[8], wd);
+    float4* f4 = reinterpret_cast<float4*>(g + 10);
+    f4[0] = make_float4((float)Mi[0], (float)Mi[1], (float)Mi[2], (float)Mi[3]);
+    f4[1] = make_float4((float)Mi[4], (float)Mi[5], (float)Mi[6], (float)Mi[7]);
+    f4[2] = make_float4((float)Mi[8], (float)wd, 0.f, 0.f);
   }
 }
 
@@ -136,7 +155,8 @@ struct PanelParams {
   const double* geo;
   int64_t E;
   int q;
-  const double* tab;   // see assemble(): Ghat / Mhat / dN / N
+  const double* tab;   // double table, see llfem_assemble
+  const float* tabf;   // FP32 copy of the same table
   double lam, mu, rho;
   double* out;
   int lds_per_wave;    // doubles
@@ -147,10 +167,94 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+template <typename Real>
+struct Geo {
+  Real Mi[9];
+  Real wd;
+};
+
+template <typename Real>
+__device__ __forceinline__ Geo<Real> load_geo(const double* __restrict__ rec) {
+  Geo<Real> g;
+  if constexpr (sizeof(Real) == 8) {
+    const double2* r2 = reinterpret_cast<const double2*>(rec);
+    double2 v0 = r2[0], v1 = r2[1], v2 = r2[2], v3 = r2[3], v4 = r2[4];
+    g.Mi[0] = v0.x; g.Mi[1] = v0.y; g.Mi[2] = v1.x; g.Mi[3] = v1.y; g.Mi[4] = v2.x;
+    g.Mi[5] = v2.y; g.Mi[6] = v3.x; g.Mi[7] = v3.y; g.Mi[8] = v4.x; g.wd = v4.y;
+  } else {
+    const float4* r4 = reinterpret_cast<const float4*>(rec + 10);
+    float4 v0 = r4[0], v1 = r4[1];
+    float2 v2 = reinterpret_cast<const float2*>(rec + 10)[4];
+    g.Mi[0] = v0.x; g.Mi[1] = v0.y; g.Mi[2] = v0.z; g.Mi[3] = v0.w; g.Mi[4] = v1.x;
+    g.Mi[5] = v1.y; g.Mi[6] = v1.z; g.Mi[7] = v1.w; g.Mi[8] = v2.x; g.wd = v2.y;
+  }
+  return g;
+}
+
+template <typename Real>
+__device__ __forceinline__ void load9(const double* d, const float* f, Real* o) {
+  if constexpr (sizeof(Real) == 8) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) o[m] = d[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) o[m] = f[m];
+  }
+}
+
+// G = Minv^T Gh Minv for one block (affine elements).
+template <typename Real>
+__device__ __forceinline__ void block_G_affine(const Geo<Real>& g, const Real* Gh, Real* G) {
+  const Real* Mi = g.Mi;
+  Real T[9];   // T[al][k] = sum_be Gh[al][be] Mi[be][k]
+#pragma unroll
+  for (int al = 0; al < 3; ++al)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      T[al * 3 + kk] = Gh[al * 3] * Mi[kk] + Gh[al * 3 + 1] * Mi[3 + kk] + Gh[al * 3 + 2] * Mi[6 + kk];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      G[i * 3 + kk] = Mi[i] * T[kk] + Mi[3 + i] * T[3 + kk] + Mi[6 + i] * T[6 + kk];
+}
+
+// K[d][c] = Ke[(b,d),(a,c)] from G and the material.
+template <bool ISO, typename Real>
+__device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Real scale, Real* K) {
+  if constexpr (ISO) {
+    const Real l = (Real)P.lam * scale, u = (Real)P.mu * scale;
+    const Real tr = u * (G[0] + G[4] + G[8]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d] + (d == c ? tr : (Real)0);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Real acc = 0;
+#pragma unroll
+        for (int al = 0; al < 3; ++al)
+#pragma unroll
+          for (int be = 0; be < 3; ++be)
+            acc += (Real)P.C[((d * 3 + al) * 3 + c) * 3 + be] * G[al * 3 + be];
+        K[d * 3 + c] = acc * scale;
+      }
+  }
+}
+
+// One wavefront owns NPW = 64/NPE node panels; lane (n, b) handles element
+// node b of node n's current incidence.  The incidence loop is software
+// pipelined: indices are fetched two steps ahead and geometry / reference
+// tables one step ahead, so the dependent load chain
+// incidence -> element -> geometry overlaps the arithmetic.
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
 __global__ __launch_bounds__(256) void k_panel(PanelParams P) {
   extern __shared__ double lds[];
-  constexpr int NPW = 64 / NPE;                 // nodes per wavefront
+  constexpr int NPW = 64 / NPE;
   constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -179,54 +283,62 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams P) {
   __syncthreads();
 
   const MapT* __restrict__ map = static_cast<const MapT*>(P.inc_map);
-  const int64_t E = P.E;
+  const int32_t* __restrict__ inc_elem = P.inc_elem;
+  const uint8_t* __restrict__ inc_loc = P.inc_loc;
+  const int q = AFFINE ? 1 : P.q;
+
+  // pipeline registers: step t (A), t+1 (B)
+  int eA = 0, aA = 0, sA = 0, eB = 0, aB = 0, sB = 0;
+  if (0 < cnt) { eA = inc_elem[k0]; aA = inc_loc[k0]; sA = map[(int64_t)k0 * NPE + b]; }
+  if (1 < cnt) { eB = inc_elem[k0 + 1]; aB = inc_loc[k0 + 1]; sB = map[(int64_t)(k0 + 1) * NPE + b]; }
+  Geo<Real> gA{};
+  Real ghA[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) ghA[m] = 0;
+  if constexpr (OP == LLFEM_OP_STIFFNESS && AFFINE) {
+    if (0 < cnt) {
+      gA = load_geo<Real>(P.geo + (size_t)eA * kGeoStride);
+      load9<Real>(P.tab + (size_t)(aA * NPE + b) * 9, P.tabf + (size_t)(aA * NPE + b) * 9, ghA);
+    }
+  }
+
   for (int t = 0; t < maxcnt; ++t) {
+    // prefetch: indices for t+2, geometry/table for t+1
+    int eC = 0, aC = 0, sC = 0;
+    if (t + 2 < cnt) {
+      const int k = k0 + t + 2;
+      eC = inc_elem[k]; aC = inc_loc[k]; sC = map[(int64_t)k * NPE + b];
+    }
+    Geo<Real> gB{};
+    Real ghB[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) ghB[m] = 0;
+    if constexpr (OP == LLFEM_OP_STIFFNESS && AFFINE) {
+      if (t + 1 < cnt) {
+        gB = load_geo<Real>(P.geo + (size_t)eB * kGeoStride);
+        load9<Real>(P.tab + (size_t)(aB * NPE + b) * 9, P.tabf + (size_t)(aB * NPE + b) * 9, ghB);
+      }
+    }
     if (t < cnt) {
-      const int k = k0 + t;
-      const int e = P.inc_elem[k];
-      const int a = P.inc_loc[k];
-      const int s = map[(int64_t)k * NPE + b];
       if constexpr (OP == LLFEM_OP_STIFFNESS) {
         Real G[9];
         Real scale;
         if constexpr (AFFINE) {
-          Real Mi[9];
-#pragma unroll
-          for (int m = 0; m < 9; ++m) Mi[m] = (Real)P.geo[m * E + e];
-          scale = (Real)P.geo[9 * E + e];
-          const double* gh = P.tab + (size_t)(b * NPE + a) * 9;
-          Real Gh[9];
-#pragma unroll
-          for (int m = 0; m < 9; ++m) Gh[m] = (Real)gh[m];
-          Real T[9];   // T[al][k] = sum_be Gh[al][be] Mi[be][k]
-#pragma unroll
-          for (int al = 0; al < 3; ++al)
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk)
-              T[al * 3 + kk] = Gh[al * 3] * Mi[kk] + Gh[al * 3 + 1] * Mi[3 + kk] +
-                               Gh[al * 3 + 2] * Mi[6 + kk];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk)
-              G[i * 3 + kk] = Mi[i] * T[kk] + Mi[3 + i] * T[3 + kk] + Mi[6 + i] * T[6 + kk];
+          block_G_affine<Real>(gA, ghA, G);
+          scale = gA.wd;
         } else {
 #pragma unroll
           for (int m = 0; m < 9; ++m) G[m] = 0;
           scale = 1;
-          for (int p = 0; p < P.q; ++p) {
-            const double* g = P.geo + (size_t)p * 10 * E;
-            Real Mi[9];
-#pragma unroll
-            for (int m = 0; m < 9; ++m) Mi[m] = (Real)g[m * E + e];
-            const Real wd = (Real)g[9 * E + e];
+          for (int p = 0; p < q; ++p) {
+            const Geo<Real> g = load_geo<Real>(P.geo + ((size_t)eA * q + p) * kGeoStride);
             const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
-            const double* ha = P.tab + ((size_t)p * NPE + a) * 3;
+            const double* ha = P.tab + ((size_t)p * NPE + aA) * 3;
             Real gb[3], ga[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              gb[i] = Mi[i] * (Real)hb[0] + Mi[3 + i] * (Real)hb[1] + Mi[6 + i] * (Real)hb[2];
-              ga[i] = (Mi[i] * (Real)ha[0] + Mi[3 + i] * (Real)ha[1] + Mi[6 + i] * (Real)ha[2]) * wd;
+              gb[i] = g.Mi[i] * (Real)hb[0] + g.Mi[3 + i] * (Real)hb[1] + g.Mi[6 + i] * (Real)hb[2];
+              ga[i] = (g.Mi[i] * (Real)ha[0] + g.Mi[3 + i] * (Real)ha[1] + g.Mi[6 + i] * (Real)ha[2]) * g.wd;
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -234,30 +346,9 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams P) {
               for (int kk = 0; kk < 3; ++kk) G[i * 3 + kk] += gb[i] * ga[kk];
           }
         }
-        Real K[9];   // K[d][c] = Ke[(b,d),(a,c)]
-        if constexpr (ISO) {
-          const Real l = (Real)P.lam * scale, u = (Real)P.mu * scale;
-          const Real tr = u * (G[0] + G[4] + G[8]);
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d] + (d == c ? tr : (Real)0);
-        } else {
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              Real acc = 0;
-#pragma unroll
-              for (int al = 0; al < 3; ++al)
-#pragma unroll
-                for (int be = 0; be < 3; ++be)
-                  acc += (Real)P.C[((d * 3 + al) * 3 + c) * 3 + be] * G[al * 3 + be];
-              K[d * 3 + c] = acc * scale;
-            }
-        }
-        double* dst = wl + base + 3 * s;
+        Real K[9];
+        block_K<ISO, Real>(P, G, scale, K);
+        double* dst = wl + base + 3 * sA;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -265,16 +356,21 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams P) {
       } else {
         double mv;
         if constexpr (AFFINE) {
-          mv = P.rho * P.geo[9 * E + e] * P.tab[b * NPE + a];
+          mv = P.rho * P.geo[(size_t)eA * kGeoStride + 9] * P.tab[aA * NPE + b];
         } else {
           mv = 0.0;
-          for (int p = 0; p < P.q; ++p)
-            mv += P.geo[((size_t)p * 10 + 9) * E + e] * P.tab[p * NPE + b] * P.tab[p * NPE + a];
+          for (int p = 0; p < q; ++p)
+            mv += P.geo[((size_t)eA * q + p) * kGeoStride + 9] * P.tab[p * NPE + b] * P.tab[p * NPE + aA];
           mv *= P.rho;
         }
-        lds_add(wl + base + s, mv);
+        lds_add(wl + base + sA, mv);
       }
     }
+    eA = eB; aA = aB; sA = sB;
+    eB = eC; aB = aC; sB = sC;
+    gA = gB;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) ghA[m] = ghB[m];
   }
   __syncthreads();
 
@@ -381,7 +477,8 @@ int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
   RuleState& rs = m->rules[key];
   const size_t geo_tab = (size_t)q * npe * 3 + q;
   const size_t op_max = std::max<size_t>((size_t)npe * npe * 9, (size_t)q * npe * 3);
-  LLFEM_TRY(ensure_buf(ctx, &m->d_tab, &m->tab_bytes, sizeof(double) * (geo_tab + op_max)));
+  // [dN | w | op table (double) | op table (float)]
+  LLFEM_TRY(ensure_buf(ctx, &m->d_tab, &m->tab_bytes, sizeof(double) * (geo_tab + op_max + op_max / 2 + 2)));
   LLFEM_HIP(hipMemcpyAsync(m->d_tab, d->dN_ref, sizeof(double) * (size_t)q * npe * 3,
                            hipMemcpyHostToDevice, st));
   LLFEM_HIP(hipMemcpyAsync(m->d_tab + (size_t)q * npe * 3, d->weights, sizeof(double) * q,
@@ -409,7 +506,7 @@ int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
 int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
   llfem_ctx* ctx = m->ctx;
   const int npe = m->npe, q = d->q;
-  const size_t geo_doubles = (size_t)10 * m->E * (affine ? 1 : q);
+  const size_t geo_doubles = (size_t)kGeoStride * m->E * (affine ? 1 : q);
   LLFEM_TRY(ensure_buf(ctx, &m->d_geo, &m->geo_bytes, sizeof(double) * geo_doubles));
   k_geometry<<<(unsigned)((m->E + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,
@@ -463,7 +560,7 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
               for (int p = 0; p < q; ++p)
                 s += d->weights[p] * d->dN_ref[((size_t)p * npe + b) * 3 + al] *
                      d->dN_ref[((size_t)p * npe + a) * 3 + be];
-              tab[((size_t)b * npe + a) * 9 + al * 3 + be] = s;
+              tab[((size_t)a * npe + b) * 9 + al * 3 + be] = s;   // [a][b] order
             }
     } else {
       tab.assign(d->dN_ref, d->dN_ref + (size_t)q * npe * 3);
@@ -483,8 +580,11 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
     }
   }
   const size_t off = (size_t)q * npe * 3 + q;
+  std::vector<float> tabf(tab.begin(), tab.end());
+  float* d_tabf = reinterpret_cast<float*>(m->d_tab + off + tab.size());
   LLFEM_HIP(hipMemcpyAsync(m->d_tab + off, tab.data(), sizeof(double) * tab.size(),
                            hipMemcpyHostToDevice, st));
+  LLFEM_HIP(hipMemcpyAsync(d_tabf, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice, st));
   LLFEM_TRY(run_geometry(m, d, affine));
 
   double** outp = (op == LLFEM_OP_STIFFNESS) ? &m->d_valK : &m->d_valM;
@@ -503,6 +603,7 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   P.E = m->E;
   P.q = q;
   P.tab = m->d_tab + off;
+  P.tabf = d_tabf;
   P.lam = 0; P.mu = 0;
   if (op == LLFEM_OP_STIFFNESS && iso) {
     // elastic_constants (src/general.jl:171-175)
