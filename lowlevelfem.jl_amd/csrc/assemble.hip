@@ -27,6 +27,7 @@
 //   * K_ba[d][c] = lam G[d][c] + mu G[c][d] + mu tr(G) delta_dc for Hooke's
 //     law, or sum C[d][al][c][be] G[al][be] for a general 6x6 D.
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -37,6 +38,7 @@ namespace llfem {
 namespace {
 
 constexpr int kMaxQ = 1024;
+constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per CU / max per workgroup
 
 // ---------------------------------------------------------------------------
 // geometry
@@ -145,13 +147,11 @@ __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
 // node-panel kernel
 // ---------------------------------------------------------------------------
 struct PanelParams {
-  const int32_t* order;
-  int64_t begin, end;
-  const int64_t* adj_ptr;
-  const int32_t* inc_ptr;
-  const int32_t* inc_elem;
-  const uint8_t* inc_loc;
-  const void* inc_map;
+  const void* rec;     // step records of the launch
+  int rec_bytes;
+  const int2* desc;    // node descriptors {adj offset, R} of the launch
+  int64_t waves;
+  int rounds, cnt, max_r;
   const double* geo;
   int64_t E;
   int q;
@@ -163,9 +163,10 @@ struct PanelParams {
   double C[81];        // general constitutive tensor C[d][al][c][be]
 };
 
-__device__ __forceinline__ void lds_add(double* p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+// Panel update: plain read-add-write.  No two lanes of a step address the
+// same live row; successive steps stay ordered because LDS executes one wave's
+// instructions in order and a compiler barrier separates the steps.
+__device__ __forceinline__ void lds_add(double* p, double v) { *p += v; }
 
 template <typename Real>
 struct Geo {
@@ -228,8 +229,10 @@ __device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Rea
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d] + (d == c ? tr : (Real)0);
+      for (int c = 0; c < 3; ++c) K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d];
+    K[0] += tr;
+    K[4] += tr;
+    K[8] += tr;
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -246,150 +249,224 @@ __device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Rea
   }
 }
 
-// One wavefront owns NPW = 64/NPE node panels; lane (n, b) handles element
-// node b of node n's current incidence.  The incidence loop is software
-// pipelined: indices are fetched two steps ahead and geometry / reference
-// tables one step ahead, so the dependent load chain
-// incidence -> element -> geometry overlaps the arithmetic.
+template <typename Real>
+struct TabBlk {
+  Real v[9];
+};
+
+// Operator-table entry of (a, b): Ghat (9 values) for stiffness, Mhat for mass.
+template <int NPE, int OP, bool AFFINE, typename Real>
+__device__ __forceinline__ TabBlk<Real> load_tab(const PanelParams& P, int a, int b) {
+  TabBlk<Real> t;
+  if constexpr (AFFINE && OP == LLFEM_OP_STIFFNESS) {
+    load9<Real>(P.tab + (size_t)(a * NPE + b) * 9, P.tabf + (size_t)(a * NPE + b) * 9, t.v);
+  } else if constexpr (AFFINE) {
+    t.v[0] = (Real)P.tab[a * NPE + b];
+  }
+  return t;
+}
+
+// Add element block Ke[(b,:),(a,:)] to the panel rows of slot s (or, for a
+// non-affine element, integrate it point by point first).
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
-__global__ __launch_bounds__(256) void k_panel(PanelParams P) {
+__device__ __forceinline__ void contribute(const PanelParams& P, const TabBlk<Real>& tb,
+                                           double* __restrict__ wl, int base, int R, int b, int e,
+                                           int a, int s, const Geo<Real>& g) {
+  if constexpr (OP == LLFEM_OP_STIFFNESS) {
+    Real G[9];
+    Real scale;
+    if constexpr (AFFINE) {
+      block_G_affine<Real>(g, tb.v, G);
+      scale = g.wd;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) G[m] = 0;
+      scale = 1;
+      for (int p = 0; p < P.q; ++p) {
+        const Geo<Real> gp = load_geo<Real>(P.geo + ((size_t)e * P.q + p) * kGeoStride);
+        const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
+        const double* ha = P.tab + ((size_t)p * NPE + a) * 3;
+        Real gb[3], ga[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          gb[i] = gp.Mi[i] * (Real)hb[0] + gp.Mi[3 + i] * (Real)hb[1] + gp.Mi[6 + i] * (Real)hb[2];
+          ga[i] = (gp.Mi[i] * (Real)ha[0] + gp.Mi[3 + i] * (Real)ha[1] + gp.Mi[6 + i] * (Real)ha[2]) * gp.wd;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) G[i * 3 + kk] += gb[i] * ga[kk];
+      }
+    }
+    Real K[9];
+    block_K<ISO, Real>(P, G, scale, K);
+    double* dst = wl + base + 3 * s;
+    double old[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) old[c * 3 + d] = dst[c * 3 * R + d];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dst[c * 3 * R + d] = old[c * 3 + d] + (double)K[d * 3 + c];
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    double mv;
+    if constexpr (AFFINE) {
+      mv = P.rho * (double)g.wd * (double)tb.v[0];
+    } else {
+      mv = 0.0;
+      for (int p = 0; p < P.q; ++p)
+        mv += P.geo[((size_t)e * P.q + p) * kGeoStride + 9] * P.tab[p * NPE + b] * P.tab[p * NPE + a];
+      mv *= P.rho;
+    }
+    lds_add(wl + base + s, mv);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Node-panel kernel over processing-order step records.
+//
+// A wavefront owns NPW = 64/NPE lane groups; lane (n, b) handles element node
+// b.  Lane group n streams `rounds` nodes one after another: in round i it
+// accumulates node p = (w*G + i)*NPW + n's panel from `cnt` step records
+// (one per incidence, null-padded), then writes the finished panel to HBM
+// and clears it.  Records of the NPW groups of one step are adjacent, so each
+// step is one coalesced 16-byte load per lane -- no pointer chasing -- and the
+// prefetch ring runs straight through round boundaries:
+//   rec[v]  step record, fetched 2*PF steps ahead;
+//   g[v]    element geometry, fetched PF steps ahead;
+//   tb[v]   reference-table block Ghat[a][b], fetched PF steps ahead.
+// All ring slot indices are compile-time (the loop is unrolled by 2*PF) and
+// every load is unconditional (clamped into the wave's record range), so the
+// waits are exact.  The only LDS traffic in a step is the panel update: a
+// plain read-add-write (volatile), conflict-free within a step because the
+// lanes of a group address distinct rows and LDS executes one wave's
+// instructions in order -- sums are deterministic and need no atomics.
+constexpr int PF = 2;
+
+struct StepRec {
+  int e, a, s;   // element, local index (-1: null step), panel row of node b
+};
+
+template <int NPE, typename MapT>
+__device__ __forceinline__ StepRec load_rec(const PanelParams& P, int64_t r, int b) {
+  StepRec o;
+  if constexpr (NPE <= 11 && sizeof(MapT) == 1) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(P.rec) + r);
+    o.e = (int)v.x;
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    const int byte = 5 + b;
+    const uint32_t a8 = v.y & 0xffu;
+    o.a = a8 == 0xffu ? -1 : (int)a8;
+    o.s = (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
+  } else {
+    const uint8_t* base = static_cast<const uint8_t*>(P.rec) + r * P.rec_bytes;
+    const int2 h = *reinterpret_cast<const int2*>(base);
+    o.e = h.x;
+    o.a = (int)(int16_t)(h.y & 0xffff);
+    o.s = (int)reinterpret_cast<const MapT*>(base + 8)[b];
+  }
+  return o;
+}
+
+template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
+__global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
   extern __shared__ double lds[];
   constexpr int NPW = 64 / NPE;
   constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
-  const int64_t first = P.begin + ((int64_t)blockIdx.x * nwaves + wave) * NPW;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (w >= P.waves) return;                       // no block-level barriers below
   double* wl = lds + (size_t)wave * P.lds_per_wave;
+  double* const dummy = wl + (size_t)NPW * P.max_r * BLK;   // 9 doubles
   const int n = lane / NPE;
   const int b = lane - n * NPE;
-  const bool has = (n < NPW) && (first + n < P.end);
-  const int j = has ? P.order[first + n] : 0;
-  const int64_t a0 = has ? P.adj_ptr[j] : 0;
-  const int R = has ? (int)(P.adj_ptr[j + 1] - a0) : 0;
-  const int k0 = has ? P.inc_ptr[j] : 0;
-  const int cnt = has ? P.inc_ptr[j + 1] - k0 : 0;
+  const bool grp = n < NPW;
+  const int G = P.rounds, cnt = P.cnt;
+  const int64_t nsteps = (int64_t)G * cnt;
+  const int64_t rbase = w * nsteps * NPW;         // first record of this wave
+  const int64_t rlast = rbase + nsteps * NPW - 1;
+  const int64_t lane_rec = grp ? n : 0;
+  double* const pan = wl + (size_t)(grp ? n : 0) * P.max_r * BLK;
 
-  int base = 0, total = 0, maxcnt = 0;
+  for (int i = lane; i < NPW * P.max_r * BLK + 9; i += 64) wl[i] = 0.0;
+
+  // node descriptors of the current and next round
+  const int2* desc = P.desc + w * (int64_t)G * NPW;
+  int2 dcur = desc[lane_rec];
+  int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
+
+  StepRec rec[2 * PF];
+  Geo<Real> g[PF];
+  TabBlk<Real> tb[PF];
 #pragma unroll
-  for (int m = 0; m < NPW; ++m) {
-    const int Rm = __shfl(R, m * NPE);
-    const int cm = __shfl(cnt, m * NPE);
-    if (m < n) base += BLK * Rm;
-    total += BLK * Rm;
-    maxcnt = cm > maxcnt ? cm : maxcnt;
+  for (int v = 0; v < 2 * PF; ++v)
+    rec[v] = load_rec<NPE, MapT>(P, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
+#pragma unroll
+  for (int v = 0; v < PF; ++v) {
+    g[v] = load_geo<Real>(P.geo + (size_t)rec[v].e * kGeoStride);
+    tb[v] = load_tab<NPE, OP, AFFINE, Real>(P, rec[v].a < 0 ? 0 : rec[v].a, b);
   }
-  for (int i = lane; i < total; i += 64) wl[i] = 0.0;
-  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 
-  const MapT* __restrict__ map = static_cast<const MapT*>(P.inc_map);
-  const int32_t* __restrict__ inc_elem = P.inc_elem;
-  const uint8_t* __restrict__ inc_loc = P.inc_loc;
-  const int q = AFFINE ? 1 : P.q;
-
-  // pipeline registers: step t (A), t+1 (B)
-  int eA = 0, aA = 0, sA = 0, eB = 0, aB = 0, sB = 0;
-  if (0 < cnt) { eA = inc_elem[k0]; aA = inc_loc[k0]; sA = map[(int64_t)k0 * NPE + b]; }
-  if (1 < cnt) { eB = inc_elem[k0 + 1]; aB = inc_loc[k0 + 1]; sB = map[(int64_t)(k0 + 1) * NPE + b]; }
-  Geo<Real> gA{};
-  Real ghA[9];
+  int t = 0, round = 0;
+  for (int64_t s0 = 0; s0 < nsteps; s0 += 2 * PF) {
 #pragma unroll
-  for (int m = 0; m < 9; ++m) ghA[m] = 0;
-  if constexpr (OP == LLFEM_OP_STIFFNESS && AFFINE) {
-    if (0 < cnt) {
-      gA = load_geo<Real>(P.geo + (size_t)eA * kGeoStride);
-      load9<Real>(P.tab + (size_t)(aA * NPE + b) * 9, P.tabf + (size_t)(aA * NPE + b) * 9, ghA);
-    }
-  }
-
-  for (int t = 0; t < maxcnt; ++t) {
-    // prefetch: indices for t+2, geometry/table for t+1
-    int eC = 0, aC = 0, sC = 0;
-    if (t + 2 < cnt) {
-      const int k = k0 + t + 2;
-      eC = inc_elem[k]; aC = inc_loc[k]; sC = map[(int64_t)k * NPE + b];
-    }
-    Geo<Real> gB{};
-    Real ghB[9];
+    for (int u = 0; u < 2 * PF; ++u) {
+      const int64_t s = s0 + u;
+      if (s < nsteps) {
+        const StepRec& cr = rec[u];
+        const bool live = grp && cr.a >= 0 && dcur.y > 0;
+        contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, tb[u % PF], live ? pan : dummy,
+                                                     0, live ? dcur.y : 0, b, cr.e,
+                                                     cr.a < 0 ? 0 : cr.a, live ? cr.s : 0,
+                                                     g[u % PF]);
+        if (++t == cnt) {   // round complete: write the NPW panels, clear them
+          t = 0;
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int m = 0; m < 9; ++m) ghB[m] = 0;
-    if constexpr (OP == LLFEM_OP_STIFFNESS && AFFINE) {
-      if (t + 1 < cnt) {
-        gB = load_geo<Real>(P.geo + (size_t)eB * kGeoStride);
-        load9<Real>(P.tab + (size_t)(aB * NPE + b) * 9, P.tabf + (size_t)(aB * NPE + b) * 9, ghB);
-      }
-    }
-    if (t < cnt) {
-      if constexpr (OP == LLFEM_OP_STIFFNESS) {
-        Real G[9];
-        Real scale;
-        if constexpr (AFFINE) {
-          block_G_affine<Real>(gA, ghA, G);
-          scale = gA.wd;
-        } else {
-#pragma unroll
-          for (int m = 0; m < 9; ++m) G[m] = 0;
-          scale = 1;
-          for (int p = 0; p < q; ++p) {
-            const Geo<Real> g = load_geo<Real>(P.geo + ((size_t)eA * q + p) * kGeoStride);
-            const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
-            const double* ha = P.tab + ((size_t)p * NPE + aA) * 3;
-            Real gb[3], ga[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              gb[i] = g.Mi[i] * (Real)hb[0] + g.Mi[3 + i] * (Real)hb[1] + g.Mi[6 + i] * (Real)hb[2];
-              ga[i] = (g.Mi[i] * (Real)ha[0] + g.Mi[3 + i] * (Real)ha[1] + g.Mi[6 + i] * (Real)ha[2]) * g.wd;
+          for (int m = 0; m < NPW; ++m) {
+            const int Rm = __shfl(dcur.y, m * NPE);
+            const int64_t om = (int64_t)__shfl(dcur.x, m * NPE);
+            double* pm = wl + (size_t)m * P.max_r * BLK;
+            if constexpr (OP == LLFEM_OP_STIFFNESS) {
+              double* o = P.out + 9 * om;
+              for (int i = lane; i < 9 * Rm; i += 64) {
+                __builtin_nontemporal_store(pm[i], o + i);
+                pm[i] = 0.0;
+              }
+            } else {
+              double* o = P.out + 3 * om;
+              for (int i = lane; i < 3 * Rm; i += 64) {
+                const int c = i / Rm;
+                __builtin_nontemporal_store(pm[i - c * Rm], o + i);
+              }
             }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk) G[i * 3 + kk] += gb[i] * ga[kk];
           }
+          if constexpr (OP != LLFEM_OP_STIFFNESS) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int i = lane; i < NPW * P.max_r; i += 64) wl[i] = 0.0;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          ++round;
+          dcur = dnext;
+          dnext = desc[(int64_t)min(round + 1, G - 1) * NPW + lane_rec];
         }
-        Real K[9];
-        block_K<ISO, Real>(P, G, scale, K);
-        double* dst = wl + base + 3 * sA;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) lds_add(dst + c * 3 * R + d, (double)K[d * 3 + c]);
-      } else {
-        double mv;
-        if constexpr (AFFINE) {
-          mv = P.rho * P.geo[(size_t)eA * kGeoStride + 9] * P.tab[aA * NPE + b];
-        } else {
-          mv = 0.0;
-          for (int p = 0; p < q; ++p)
-            mv += P.geo[((size_t)eA * q + p) * kGeoStride + 9] * P.tab[p * NPE + b] * P.tab[p * NPE + aA];
-          mv *= P.rho;
-        }
-        lds_add(wl + base + sA, mv);
       }
+      // ring refill: geometry/table for step s+PF, record for step s+2PF
+      const StepRec& nx = rec[(u + PF) % (2 * PF)];
+      g[u % PF] = load_geo<Real>(P.geo + (size_t)nx.e * kGeoStride);
+      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(P, nx.a < 0 ? 0 : nx.a, b);
+      rec[u] = load_rec<NPE, MapT>(P, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
     }
-    eA = eB; aA = aB; sA = sB;
-    eB = eC; aB = aC; sB = sC;
-    gA = gB;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) ghA[m] = ghB[m];
-  }
-  __syncthreads();
-
-  int bm = 0;
-#pragma unroll
-  for (int m = 0; m < NPW; ++m) {
-    const int Rm = __shfl(R, m * NPE);
-    const long long om = __shfl((long long)a0, m * NPE);
-    if constexpr (OP == LLFEM_OP_STIFFNESS) {
-      double* o = P.out + 9 * om;
-      for (int i = lane; i < 9 * Rm; i += 64) o[i] = wl[bm + i];
-    } else {
-      double* o = P.out + 3 * om;
-      for (int i = lane; i < 3 * Rm; i += 64) {
-        const int c = i / Rm;
-        o[i] = wl[bm + i - c * Rm];
-      }
-    }
-    bm += BLK * Rm;
   }
 }
 
@@ -593,12 +670,7 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   if (!*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)nnz));
 
   PanelParams P{};
-  P.order = m->d_order;
-  P.adj_ptr = m->d_adj_ptr;
-  P.inc_ptr = m->d_inc_ptr;
-  P.inc_elem = m->d_inc_elem;
-  P.inc_loc = m->d_inc_loc;
-  P.inc_map = m->d_inc_map;
+  P.rec_bytes = m->rec_bytes;
   P.geo = m->d_geo;
   P.E = m->E;
   P.q = q;
@@ -621,18 +693,33 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   const int npw = 64 / npe;
   const int blkl = (op == LLFEM_OP_STIFFNESS) ? 9 : 1;
   LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
-  for (const NodeClass& c : m->classes) {
-    P.begin = c.begin;
-    P.end = c.end;
-    P.lds_per_wave = npw * c.max_r * blkl;
+  for (const PanelLaunch& L : m->launches) {
+    P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
+    P.desc = m->d_desc + L.desc_base;
+    P.waves = L.waves;
+    P.rounds = L.rounds;
+    P.cnt = L.cnt;
+    P.max_r = L.max_r;
+    P.lds_per_wave = npw * L.max_r * blkl + 9;   // + dummy block
     const size_t wave_bytes = sizeof(double) * (size_t)P.lds_per_wave;
-    int wpb = (int)std::min<size_t>(4, std::max<size_t>(1, 65536 / std::max<size_t>(wave_bytes, 1)));
-    if (wave_bytes * wpb > 160 * 1024)
-      return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
-    const int64_t nodes = c.end - c.begin;
-    const int64_t waves = (nodes + npw - 1) / npw;
-    const unsigned blocks = (unsigned)((waves + wpb - 1) / wpb);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * wpb), wave_bytes * wpb, st, P);
+    size_t max_wpb = (iso || op == LLFEM_OP_MASS) ? 8 : 4;   // matches k_panel launch bounds
+    if (const char* env = std::getenv("LLFEM_PANEL_MAX_WAVES"))     // tuning knob
+      max_wpb = std::max<size_t>(1, std::min<size_t>(max_wpb, (size_t)std::atoi(env)));
+    // waves per block: the size that packs the most waves into one CU's LDS
+    // (several small blocks beat one large block with LDS left over)
+    int wpb = 0;
+    size_t best = 0;
+    for (size_t c = 1; c <= max_wpb && c * wave_bytes <= kLdsPerCU; ++c) {
+      const size_t resident = c * (kLdsPerCU / (c * wave_bytes));
+      if (resident > best) { best = resident; wpb = (int)c; }
+    }
+    if (wpb < 1) return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
+    const unsigned blocks = (unsigned)((L.waves + wpb - 1) / wpb);
+    const size_t lds_bytes = wave_bytes * wpb;
+    if (lds_bytes > 65536)
+      LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_bytes));
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * wpb), lds_bytes, st, P);
   }
   LLFEM_HIP(hipGetLastError());
   LLFEM_HIP(hipEventRecord(ctx->ev[2], st));

@@ -37,12 +37,17 @@ inline int fail(int code, const std::string& msg) {
   return code;
 }
 
-// Nodes that share an incidence count form one launch class (no divergence
-// in the incidence loop); max_r sizes the LDS panels of the class.
-struct NodeClass {
-  int64_t begin, end;   // range in the processing order
-  int32_t count;        // incidences per node
-  int32_t max_r;        // largest node-adjacency row count in the class
+// One panel-kernel launch.  Nodes of similar incidence count are grouped
+// (padded with null steps to `cnt`), laid out in processing order as
+// per-step records so a wavefront streams `rounds` nodes per lane group.
+struct PanelLaunch {
+  int64_t rec_base;     // first record of the launch
+  int64_t desc_base;    // first node descriptor of the launch
+  int64_t waves;        // wavefronts in the launch
+  int32_t cnt;          // steps (incidences, padded) per node
+  int32_t rounds;       // nodes per lane group (per wavefront stream)
+  int32_t max_r;        // LDS panel rows per lane group
+  int32_t nodes;        // real nodes in the launch
 };
 
 // Per-rule geometry state: affine detection is a property of the mesh and
@@ -83,8 +88,13 @@ struct llfem_mesh {
   uint8_t* d_inc_loc = nullptr;   // [ninc]
   void* d_inc_map = nullptr;      // [ninc][npe] slot of each element node in the panel
   int32_t map_bytes = 1;
-  int32_t* d_order = nullptr;     // [N] nodes sorted by incidence count
-  std::vector<llfem::NodeClass> classes;
+  // processing-order step records + node descriptors (see PanelLaunch)
+  void* d_rec = nullptr;
+  int64_t rec_count = 0;
+  int32_t rec_bytes = 16;
+  int2* d_desc = nullptr;         // {adj offset, R} per node slot
+  int64_t desc_count = 0;
+  std::vector<llfem::PanelLaunch> launches;
   int32_t max_r = 0;
   // host copies kept for pattern export
   std::vector<int64_t> h_adj_ptr;

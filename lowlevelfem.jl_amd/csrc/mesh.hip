@@ -135,7 +135,153 @@ int sort_keys(llfem_ctx* ctx, uint64_t*& keys, uint64_t*& tmp, int64_t n, int en
   return LLFEM_OK;
 }
 
+// One record per (wavefront, round, step, lane group):
+//   r = ((w*G + i)*cnt + t)*NPW + n   <->   node slot p = (w*G + i)*NPW + n.
+// Compact record (npe <= 11, byte slots): {int32 e; uint8 a; uint8 slot[11]}.
+// Generic record: {int32 e; int16 a; int16 pad; MapT slot[npe]} padded to 16 B.
+// Null steps (node has fewer incidences than cnt, or padding node) carry
+// a = all-ones and element 0.
+template <typename MapT>
+__global__ void k_build_records(int64_t nrec, int npe, int npw, int cnt, int rec_bytes,
+                                int compact, int64_t nodes, const int32_t* __restrict__ order,
+                                const int32_t* __restrict__ inc_ptr,
+                                const int32_t* __restrict__ inc_elem,
+                                const uint8_t* __restrict__ inc_loc, const MapT* __restrict__ map,
+                                const int64_t* __restrict__ adj_ptr, uint8_t* __restrict__ rec,
+                                int2* __restrict__ desc) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int n = (int)(r % npw);
+  const int64_t q1 = r / npw;
+  const int t = (int)(q1 % cnt);
+  const int64_t p = (q1 / cnt) * npw + n;
+  int32_t e = 0, a = -1;
+  int64_t k = -1;
+  int32_t j = -1;
+  if (p < nodes) {
+    j = order[p];
+    const int32_t k0 = inc_ptr[j];
+    if (t < inc_ptr[j + 1] - k0) {
+      k = k0 + t;
+      e = inc_elem[k];
+      a = inc_loc[k];
+    }
+  }
+  uint8_t* o = rec + r * rec_bytes;
+  for (int i = 0; i < rec_bytes; ++i) o[i] = 0;
+  *reinterpret_cast<int32_t*>(o) = e;
+  if (compact) {
+    o[4] = (uint8_t)(a < 0 ? 0xff : a);
+    if (k >= 0)
+      for (int b = 0; b < npe; ++b) o[5 + b] = (uint8_t)map[k * npe + b];
+  } else {
+    *reinterpret_cast<int16_t*>(o + 4) = (int16_t)a;
+    MapT* sl = reinterpret_cast<MapT*>(o + 8);
+    if (k >= 0)
+      for (int b = 0; b < npe; ++b) sl[b] = map[k * npe + b];
+  }
+  if (t == 0) {
+    int2 d = make_int2(0, 0);
+    if (j >= 0) d = make_int2((int32_t)adj_ptr[j], (int32_t)(adj_ptr[j + 1] - adj_ptr[j]));
+    desc[p] = d;
+  }
+}
+
 }  // namespace
+
+// Plan the panel launches and build the processing-order records.
+// Nodes are sorted by incidence count; consecutive counts are merged into one
+// launch while the padding waste stays small (or the class is tiny), so a
+// mesh needs only a handful of launches.
+int build_records(llfem_mesh* m, const std::vector<int32_t>& h_inc_ptr) {
+  llfem_ctx* ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t N = m->N;
+  const int npe = m->npe;
+  const int npw = 64 / npe;
+  m->launches.clear();
+  if (npw < 1) return LLFEM_OK;   // npe > 64: no node panels (high-order path)
+  auto count = [&](int64_t j) { return h_inc_ptr[j + 1] - h_inc_ptr[j]; };
+  auto rows = [&](int64_t j) { return (int32_t)(m->h_adj_ptr[j + 1] - m->h_adj_ptr[j]); };
+  std::vector<int32_t> order;
+  order.reserve(N);
+  for (int64_t j = 0; j < N; ++j)
+    if (count(j) > 0) order.push_back((int32_t)j);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return count(x) < count(y); });
+  const bool compact = (npe <= 11 && m->map_bytes == 1);
+  m->rec_bytes = compact ? 16 : (int)(((8 + (int64_t)npe * m->map_bytes) + 15) / 16 * 16);
+  m->launches.clear();
+  int64_t rec_total = 0, desc_total = 0;
+  std::vector<std::pair<int64_t, int64_t>> ranges;   // order ranges of the launches
+  const int64_t ncu = std::max(1, ctx->num_cus);
+  for (int64_t p = 0; p < (int64_t)order.size();) {
+    const int32_t cmin = count(order[p]);
+    int64_t q = p;
+    int32_t cmax = cmin, rmax = 0;
+    while (q < (int64_t)order.size()) {
+      const int32_t c = count(order[q]);
+      // merge while the padded steps stay within 25% of the smallest count,
+      // or while the launch is still tiny
+      if (c > cmax && c * 4 > cmin * 5 && (q - p) >= 4096) break;
+      cmax = std::max(cmax, c);
+      rmax = std::max(rmax, rows(order[q]));
+      ++q;
+    }
+    const int64_t nodes = q - p;
+    const int blk = 9;   // LDS sized for stiffness; mass uses less
+    const int64_t wave_bytes = 8 * ((int64_t)npw * rmax * blk + 9);
+    const int64_t wpc = std::max<int64_t>(1, std::min<int64_t>(16, (160 * 1024) / wave_bytes));
+    const int64_t target = 8 * ncu * wpc;                       // wavefronts per launch
+    int64_t G = (nodes + npw * target - 1) / (npw * target);
+    G = std::max<int64_t>(1, std::min<int64_t>(G, 256));
+    const int64_t per_wave = npw * G;
+    const int64_t waves = (nodes + per_wave - 1) / per_wave;
+    const int64_t slots = waves * per_wave;
+    PanelLaunch L;
+    L.rec_base = rec_total;
+    L.desc_base = desc_total;
+    L.waves = waves;
+    L.cnt = cmax;
+    L.rounds = (int32_t)G;
+    L.max_r = rmax;
+    L.nodes = (int32_t)nodes;
+    m->launches.push_back(L);
+    ranges.push_back({p, q});
+    rec_total += slots * cmax;
+    desc_total += slots;
+    p = q;
+  }
+  m->rec_count = rec_total;
+  m->desc_count = desc_total;
+  LLFEM_TRY(dmalloc(ctx, &m->d_rec, (size_t)rec_total * m->rec_bytes));
+  LLFEM_TRY(dalloc(ctx, &m->d_desc, (size_t)desc_total));
+  int32_t* d_order = nullptr;
+  LLFEM_TRY(dalloc(ctx, &d_order, order.size() + 1));
+  LLFEM_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice, st));
+  for (size_t li = 0; li < m->launches.size(); ++li) {
+    const PanelLaunch& L = m->launches[li];
+    const int64_t nrec = L.waves * npw * L.rounds * (int64_t)L.cnt;
+    uint8_t* rec = static_cast<uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
+    const unsigned g = (unsigned)((nrec + 255) / 256);
+    if (m->map_bytes == 1)
+      k_build_records<uint8_t><<<g, 256, 0, st>>>(nrec, npe, npw, L.cnt, m->rec_bytes, compact ? 1 : 0,
+                                                  L.nodes, d_order + ranges[li].first, m->d_inc_ptr,
+                                                  m->d_inc_elem, m->d_inc_loc,
+                                                  (const uint8_t*)m->d_inc_map, m->d_adj_ptr, rec,
+                                                  m->d_desc + L.desc_base);
+    else
+      k_build_records<uint16_t><<<g, 256, 0, st>>>(nrec, npe, npw, L.cnt, m->rec_bytes, 0, L.nodes,
+                                                   d_order + ranges[li].first, m->d_inc_ptr,
+                                                   m->d_inc_elem, m->d_inc_loc,
+                                                   (const uint16_t*)m->d_inc_map, m->d_adj_ptr, rec,
+                                                   m->d_desc + L.desc_base);
+  }
+  LLFEM_HIP(hipGetLastError());
+  LLFEM_HIP(hipStreamSynchronize(st));
+  drelease(ctx, d_order, order.size() + 1);
+  return LLFEM_OK;
+}
 
 int build_pattern(llfem_mesh* m) {
   llfem_ctx* ctx = m->ctx;
@@ -203,28 +349,6 @@ int build_pattern(llfem_mesh* m) {
   m->map_bytes = max_r <= 256 ? 1 : 2;
   if (max_r > 65535) return fail(LLFEM_EUNSUP, "node with more than 65535 neighbours");
 
-  std::vector<int32_t> order(N);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-    int32_t cx = h_inc_ptr[x + 1] - h_inc_ptr[x], cy = h_inc_ptr[y + 1] - h_inc_ptr[y];
-    return cx < cy;
-  });
-  m->classes.clear();
-  for (int64_t p = 0; p < N;) {
-    int32_t c = h_inc_ptr[order[p] + 1] - h_inc_ptr[order[p]];
-    int64_t q = p;
-    int32_t mr = 0;
-    while (q < N && h_inc_ptr[order[q] + 1] - h_inc_ptr[order[q]] == c) {
-      int64_t j = order[q];
-      mr = std::max<int32_t>(mr, (int32_t)(m->h_adj_ptr[j + 1] - m->h_adj_ptr[j]));
-      ++q;
-    }
-    if (c > 0) m->classes.push_back({p, q, c, mr});
-    p = q;
-  }
-  LLFEM_TRY(dalloc(ctx, &m->d_order, N));
-  LLFEM_HIP(hipMemcpyAsync(m->d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-
   // ---- element->nnz map ------------------------------------------------------
   int* d_err = nullptr;
   LLFEM_TRY(dalloc(ctx, &d_err, 1));
@@ -240,6 +364,7 @@ int build_pattern(llfem_mesh* m) {
                                                      (uint16_t*)m->d_inc_map, d_err);
   LLFEM_TRY(dalloc(ctx, &m->d_diag_slot, N));
   k_diag_slot<<<grid(N), 256, 0, st>>>(N, m->d_adj_ptr, m->d_adj, m->d_diag_slot);
+  LLFEM_TRY(build_records(m, h_inc_ptr));
   int herr = 0;
   LLFEM_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
   LLFEM_HIP(hipStreamSynchronize(st));
@@ -278,7 +403,9 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_inc_loc, m->ninc);
   if (m->d_inc_map) dfree(c, m->d_inc_map, (size_t)m->ninc * m->npe * m->map_bytes);
   m->d_inc_map = nullptr;
-  drelease(c, m->d_order, m->N);
+  if (m->d_rec) dfree(c, m->d_rec, (size_t)m->rec_count * m->rec_bytes);
+  m->d_rec = nullptr;
+  drelease(c, m->d_desc, m->desc_count);
   if (m->d_geo) dfree(c, m->d_geo, m->geo_bytes);
   if (m->d_tab) dfree(c, m->d_tab, m->tab_bytes);
   drelease(c, m->d_valK, 9 * m->nadj);
