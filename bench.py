@@ -63,6 +63,36 @@ def cpu_baseline(cells):
                       f"{T} OpenMP threads on {cpu_model()}"}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "panel_assembly_tet10_beam_40x40x160.md")
+
+
+def measured_traffic():
+    """HBM bytes per assembly from the committed rocprofv3 FETCH/WRITE summary
+    of this workload (null when the summary is absent)."""
+    try:
+        for line in open(PROFILE):
+            if "traffic_bytes_per_assembly:" in line:
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def calib_copy(nbytes):
+    """One device-to-device hipMemcpy of nbytes (HIP's blit kernel): the
+    reference dispatch whose FETCH_SIZE/WRITE_SIZE scale the counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +100,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default=None, help="override nx,ny,nz (testing only)")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="after the steps, copy this many bytes device-to-device once "
+                         "(reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -130,13 +163,16 @@ def main():
                                "1x1x4 box, 4-point rule, FP64 stiffness into CSC",
                    "elements": m.E, "nodes": m.N, "dofs": 3 * m.N, "nnz": nnz},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if cells == CELLS else None},
         "kernel_ms": {"assembly_events": k_ms, "panel_kernels_events": p_ms,
                       "geometry_events": k_ms - p_ms},
         "fp32_integrate_fp64_sum": {"kernel_ms": st32.kernel_ms,
                                     "elem_per_s": m.E / (st32.kernel_ms * 1e-3)},
         "setup_s": setup_s,
     }
+    if args.calib_copy:
+        calib_copy(args.calib_copy)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline((40, 40, 16) if not args.cells else cells)
     print(json.dumps(out), flush=True)

@@ -39,6 +39,7 @@ namespace {
 
 constexpr int kMaxQ = 1024;
 constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per CU / max per workgroup
+constexpr int kXcds = 8;                    // MI355X accelerator complex dies
 
 // ---------------------------------------------------------------------------
 // geometry
@@ -379,7 +380,15 @@ __global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
   constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  // XCD-aware work order: the dispatcher deals blocks round-robin to the 8
+  // XCDs, so block b runs on XCD b % 8.  Give XCD x the contiguous block range
+  // [x*per, (x+1)*per) of the processing order; neighbouring node groups then
+  // share an L2 and re-read element geometry from it rather than from HBM.
+  // The host pads the grid to a multiple of kXcds, which makes the remap a
+  // bijection of [0, gridDim.x); padding blocks fail the w < waves test.
+  const int64_t per = gridDim.x / kXcds;
+  const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;   // logical block
+  const int64_t w = lb * (blockDim.x >> 6) + wave;
   if (w >= P.waves) return;                       // no block-level barriers below
   double* wl = lds + (size_t)wave * P.lds_per_wave;
   double* const dummy = wl + (size_t)NPW * P.max_r * BLK;   // 9 doubles
@@ -714,7 +723,8 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
       if (resident > best) { best = resident; wpb = (int)c; }
     }
     if (wpb < 1) return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
-    const unsigned blocks = (unsigned)((L.waves + wpb - 1) / wpb);
+    const int64_t used = (L.waves + wpb - 1) / wpb;
+    const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
     const size_t lds_bytes = wave_bytes * wpb;
     if (lds_bytes > 65536)
       LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
