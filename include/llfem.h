@@ -88,6 +88,7 @@ typedef struct {
   int64_t iterations;      /* solver: total (inner) CG iterations */
   int64_t refinements;     /* solver: FP64 refinement steps */
   double relres;           /* solver: ||(K u - f)_free|| / ||f_free|| */
+  double alg_flops;        /* element-integration flops of the call (see llfem_assemble) */
 } llfem_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -128,7 +129,12 @@ LLFEM_API int llfem_mesh_pattern(llfem_mesh* mesh, int32_t op, int32_t one_based
  * massMatrix(problem; lumped) (src/linear.jl:1843-2004).
  * The values stay resident on the device (reused by llfem_solve); when
  * nzval_out != NULL they are also copied to the host, in the order of
- * llfem_mesh_pattern(op) (for lumped mass: the pdim*N diagonal). */
+ * llfem_mesh_pattern(op) (for lumped mass: the pdim*N diagonal).
+ * Stiffness of hex125 (and hex64 with LLFEM_BF16X3, or LLFEM_HIGHORDER=1 in
+ * the environment) runs as a per-element Gram product on the matrix cores
+ * (FP64 MFMA, or BF16x3 with FP32 accumulation); those need positive
+ * quadrature weights and q <= 128.  FP32_INT_FP64_SUM on that path
+ * computes in FP64. */
 LLFEM_API int llfem_assemble(llfem_mesh* mesh, const llfem_op_desc* desc, double* nzval_out,
                    llfem_stats* stats);
 

@@ -614,10 +614,17 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
     return fail(LLFEM_EINVAL, "llfem_assemble: bad quadrature tables");
   if (d->op == LLFEM_OP_MASS && !d->N_ref)
     return fail(LLFEM_EINVAL, "llfem_assemble: mass needs N_ref");
-  if (d->precision == LLFEM_BF16X3)
-    return fail(LLFEM_EUNSUP, "llfem_assemble: BF16x3 is for order >= 3 hexahedra");
-  if (d->precision != LLFEM_FP64 && d->precision != LLFEM_FP32_INT_FP64_SUM)
+  if (d->precision != LLFEM_FP64 && d->precision != LLFEM_FP32_INT_FP64_SUM &&
+      d->precision != LLFEM_BF16X3)
     return fail(LLFEM_EINVAL, "llfem_assemble: unknown precision");
+  // order >= 3 hexahedra (hex64, hex125): Gram formulation on the matrix cores.
+  // hex64 FP64 stays on the panel kernel unless LLFEM_HIGHORDER=1.
+  const bool ho_type = (m->etype == 92 || m->etype == 93) && d->op == LLFEM_OP_STIFFNESS;
+  bool highorder = ho_type && (m->npe > 64 || d->precision == LLFEM_BF16X3);
+  if (ho_type)
+    if (const char* env = std::getenv("LLFEM_HIGHORDER")) highorder = highorder || std::atoi(env) != 0;
+  if (d->precision == LLFEM_BF16X3 && !highorder)
+    return fail(LLFEM_EUNSUP, "llfem_assemble: BF16x3 is for order >= 3 hexahedra (stiffness)");
   llfem_ctx* ctx = m->ctx;
   LLFEM_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -629,13 +636,15 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   bool affine = false;
   LLFEM_TRY(geometry_state(m, d, &affine));
 
-  PanelFn fn = select_panel(npe, m->map_bytes, op, affine, iso, fp32);
-  if (!fn)
+  PanelFn fn = highorder ? nullptr : select_panel(npe, m->map_bytes, op, affine, iso, fp32);
+  if (!fn && !highorder)
     return fail(LLFEM_EUNSUP, "llfem_assemble: no panel kernel for npe=" + std::to_string(npe));
 
   // per-operator reference tables (host, FP64)
   std::vector<double> tab;
-  if (op == LLFEM_OP_STIFFNESS) {
+  if (highorder) {
+    // the Gram kernels read dN and w directly
+  } else if (op == LLFEM_OP_STIFFNESS) {
     if (affine) {
       tab.assign((size_t)npe * npe * 9, 0.0);
       for (int b = 0; b < npe; ++b)
@@ -701,8 +710,12 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   P.out = *outp;
   const int npw = 64 / npe;
   const int blkl = (op == LLFEM_OP_STIFFNESS) ? 9 : 1;
-  LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
-  for (const PanelLaunch& L : m->launches) {
+  if (highorder)
+    LLFEM_TRY(assemble_highorder(m, d, affine, m->d_tab, m->d_tab + (size_t)q * npe * 3, P.lam, P.mu,
+                                 iso ? nullptr : P.C, *outp));
+  else
+    LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
+  for (const PanelLaunch& L : (highorder ? std::vector<PanelLaunch>() : m->launches)) {
     P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
     P.desc = m->d_desc + L.desc_base;
     P.waves = L.waves;
@@ -732,7 +745,7 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * wpb), lds_bytes, st, P);
   }
   LLFEM_HIP(hipGetLastError());
-  LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
+  if (!highorder) LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
   if (op == LLFEM_OP_MASS && d->lumped) {
     if (!m->d_lumped) LLFEM_TRY(dalloc(ctx, &m->d_lumped, 3 * m->N));
     k_lump<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_valM, m->d_lumped);
@@ -760,6 +773,12 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
     stats->nnz = nnz;
     stats->elements = m->E;
     stats->alg_bytes = 24.0 * m->N + 4.0 * npe * m->E + 8.0 * nnz;
+    // element integration flops: symmetric Gram (3n)(3n+1) q on the high-order
+    // path; on the panel path 2*9 q flops per (a,b) block pair for non-affine
+    // rules, 0 (table lookup) for affine ones
+    const double n3 = 3.0 * npe;
+    stats->alg_flops = highorder ? n3 * (n3 + 1.0) * q * (double)m->E
+                                 : (affine ? 0.0 : 18.0 * q * (double)npe * npe * (double)m->E);
     stats->iterations = 0;
     stats->refinements = 0;
     stats->relres = 0;

@@ -113,6 +113,14 @@ struct llfem_mesh {
   float* d_valK32 = nullptr;
   double* d_diagK = nullptr;
   bool solver_ready = false;
+  // staged high-order assembly: per nnz block, its element blocks in
+  // ascending element order (d_csrc[d_cptr[p] .. d_cptr[p+1]))
+  int64_t ncontrib = 0;
+  int64_t* d_cptr = nullptr;      // [nadj+1]
+  int32_t* d_csrc = nullptr;      // [ncontrib] staged block index e*npe^2 + a*npe + b
+  int32_t* d_blk_node = nullptr;  // [nadj] column node of each nnz block
+  void* d_ho = nullptr;           // staged blocks + Z
+  size_t ho_bytes = 0;
 };
 
 namespace llfem {
@@ -140,6 +148,9 @@ inline int etype_dim(int et) {
     default: return 0;
   }
 }
+
+int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const double* d_dN,
+                       const double* d_w, double lam, double mu, const double* C81, double* out);
 
 uint64_t hash_bytes(const void* p, size_t n, uint64_t h = 1469598103934665603ull);
 

@@ -413,6 +413,10 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_lumped, 3 * m->N);
   drelease(c, m->d_valK32, 9 * m->nadj);
   drelease(c, m->d_diagK, 3 * m->N);
+  drelease(c, m->d_cptr, m->nadj + 1);
+  drelease(c, m->d_csrc, m->ncontrib);
+  drelease(c, m->d_blk_node, m->nadj);
+  if (m->d_ho) dfree(c, m->d_ho, m->ho_bytes);
 }
 
 extern "C" {

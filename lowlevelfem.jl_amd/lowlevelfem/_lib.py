@@ -36,7 +36,8 @@ class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
                 ("total_ms", C.c_double), ("nnz", C.c_int64), ("elements", C.c_int64),
                 ("alg_bytes", C.c_double), ("iterations", C.c_int64),
-                ("refinements", C.c_int64), ("relres", C.c_double)]
+                ("refinements", C.c_int64), ("relres", C.c_double),
+                ("alg_flops", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
