@@ -27,6 +27,7 @@ constexpr int kHoTile = 96;     // G tile: 96 rows = 32 nodes x 3 components
 constexpr int kHoMaxRows = 384; // padded 3n: n <= 128
 constexpr int kHoMaxQ = 128;    // padded number of points
 constexpr int kGeoRec = 16;     // doubles per geometry record written by k_geometry
+constexpr int kHoXcds = 8;      // XCDs of the device (blocks are dealt round-robin)
 
 namespace {
 
@@ -91,6 +92,7 @@ struct GramParams {
   int64_t E;
   int npe;
   int ntiles;             // node tiles of 32 (rows = 96 * ntiles)
+  int npairs;             // upper tile pairs per element
   int qp;                 // padded points
   const double* Z;
   const short* Zh;
@@ -108,26 +110,39 @@ __device__ __forceinline__ void tile_pair(int idx, int nt, int& ti, int& tj) {
   tj = ti + idx;
 }
 
-// Epilogue shared by both precisions: G tile in LDS -> staged K blocks.
-__device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, int ti, int tj) {
+// Block -> (element, tile pair).  Hardware deals consecutive blocks round-robin
+// over the 8 XCDs; the remap gives each XCD a contiguous run of (element,
+// pair) ids so all pairs of an element share one L2 (Z is fetched from HBM
+// once per element).  The grid is padded to a multiple of 8.
+__device__ __forceinline__ bool gram_block(const GramParams& P, int64_t& e, int& ti, int& tj) {
+  const unsigned L = blockIdx.x, nb = gridDim.x;
+  const unsigned id = (L % kHoXcds) * (nb / kHoXcds) + L / kHoXcds;
+  e = id / P.npairs;
+  if (e >= P.E) return false;
+  tile_pair((int)(id - e * P.npairs), P.ntiles, ti, tj);
+  return true;
+}
+
+// Epilogue: LDS holds rows [48h, 48h+48) of the 96x96 G tile (row nodes
+// 16h..16h+15 of tile ti, all 32 column nodes of tile tj) -> staged K blocks.
+__device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, int ti, int tj, int h) {
   const int npe = P.npe;
   double* S = P.S + (size_t)e * npe * npe * 9;
   for (int pass = 0; pass < (ti == tj ? 1 : 2); ++pass) {
-    for (int idx = threadIdx.x; idx < 32 * 32; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < 16 * 32; idx += blockDim.x) {
       // pass 0: block (row b in ti, col a in tj), b fastest across threads;
       // pass 1: its transpose (row a in tj, col b in ti), a fastest.
-      const int slow = idx >> 5, fast = idx & 31;
-      const int bl = pass == 0 ? fast : slow;   // local row node in tile ti
-      const int al = pass == 0 ? slow : fast;   // local col node in tile tj
-      const int b = ti * 32 + bl, a = tj * 32 + al;
+      const int hl = pass == 0 ? (idx & 15) : (idx >> 5);   // row node within the half
+      const int al = pass == 0 ? (idx >> 4) : (idx & 31);   // col node in tile tj
+      const int b = ti * 32 + 16 * h + hl, a = tj * 32 + al;
       if (b >= npe || a >= npe) continue;
-      // G_ba[al][be] of (row node b, col node a); pass 1 needs G_ab = G_ba^T
+      // G_ba[x][y] of (row node b, col node a); pass 1 needs G_ab = G_ba^T
       double G[9];
 #pragma unroll
       for (int x = 0; x < 3; ++x)
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
-          const double g = Gt[(3 * bl + x) * kHoTile + 3 * al + y];
+          const double g = Gt[(3 * hl + x) * kHoTile + 3 * al + y];
           if (pass == 0) G[x * 3 + y] = g; else G[y * 3 + x] = g;
         }
       double K[9];
@@ -163,99 +178,131 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, 
 }
 
 // One block per (element, upper tile pair); 4 waves, each a 3x3 grid of 16x16
-// subtiles of the 96x96 tile (wave w: row blocks 3*(w/2).., col blocks 3*(w%2)..).
-__global__ __launch_bounds__(256) void k_gram_f64(GramParams P) {
-  extern __shared__ double Gt[];   // 96 x 96
-  const int64_t e = blockIdx.x;
+// subtiles of the 96x96 tile (wave w: rows 48*(w/2).., cols 48*(w%2)..).
+// Fragments of step kb+1 are loaded while the MFMAs of step kb run.  The tile
+// goes through LDS in two 48-row halves (36 KB), so four blocks fit a CU.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_gram_f64(GramParams P) {
+  extern __shared__ double Gt[];   // 48 x 96
+  int64_t e;
   int ti, tj;
-  tile_pair(blockIdx.y, P.ntiles, ti, tj);
+  if (!gram_block(P, e, ti, tj)) return;   // block-uniform
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r0 = ti * kHoTile + (wave >> 1) * 48;
-  const int c0 = tj * kHoTile + (wave & 1) * 48;
+  const int rows = P.ntiles * kHoTile;
+  const int lr = lane & 15, lk = lane >> 4;
+  const double* Za = P.Z + (size_t)e * P.qp * rows + (size_t)(ti * kHoTile + (wave >> 1) * 48 + lr) * 4 + lk;
+  const double* Zb = P.Z + (size_t)e * P.qp * rows + (size_t)(tj * kHoTile + (wave & 1) * 48 + lr) * 4 + lk;
+  const size_t kstride = (size_t)rows * 4;
+  const int nk = P.qp / 4;   // even: qp is a multiple of 16
   d4 acc[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int rows = P.ntiles * kHoTile;
-  const double* Ze = P.Z + (size_t)e * P.qp * rows;
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int kb = 0; kb < P.qp / 4; ++kb) {
-    const double* Zk = Ze + (size_t)kb * rows * 4;
-    double a[3], b[3];
+  double a0[3], b0[3], a1[3], b1[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      a[i] = Zk[(r0 + 16 * i + lr) * 4 + lk];
-      b[i] = Zk[(c0 + 16 * i + lr) * 4 + lk];
-    }
+  for (int i = 0; i < 3; ++i) { a0[i] = Za[64 * i]; b0[i] = Zb[64 * i]; }
+  for (int kb = 0; kb < nk; kb += 2) {
+    const size_t o1 = (size_t)(kb + 1) * kstride;
+    const size_t o2 = (size_t)(kb + 2 < nk ? kb + 2 : nk - 1) * kstride;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { a1[i] = Za[o1 + 64 * i]; b1[i] = Zb[o1 + 64 * i]; }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i], b0[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { a0[i] = Za[o2 + 64 * i]; b0[i] = Zb[o2 + 64 * i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i], b1[j], acc[i][j], 0, 0, 0);
   }
   // f64 16x16x4 output layout: lane l, reg r -> row (l/16) + 4r, col l%16
-  const int tr0 = (wave >> 1) * 48, tc0 = (wave & 1) * 48;
+  const int tc0 = (wave & 1) * 48;
+  for (int h = 0; h < 2; ++h) {
+    if ((wave >> 1) == h) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Gt[(tr0 + 16 * i + lk + 4 * r) * kHoTile + tc0 + 16 * j + lr] = acc[i][j][r];
-  __syncthreads();
-  gram_epilogue(P, Gt, e, ti, tj);
+          for (int r = 0; r < 4; ++r)
+            Gt[(16 * i + lk + 4 * r) * kHoTile + tc0 + 16 * j + lr] = acc[i][j][r];
+    }
+    __syncthreads();
+    gram_epilogue(P, Gt, e, ti, tj, h);
+    __syncthreads();
+  }
 }
 
-__global__ __launch_bounds__(256) void k_gram_bf16x3(GramParams P) {
-  extern __shared__ double Gt[];
-  const int64_t e = blockIdx.x;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_gram_bf16x3(GramParams P) {
+  extern __shared__ double Gt[];   // 48 x 96
+  int64_t e;
   int ti, tj;
-  tile_pair(blockIdx.y, P.ntiles, ti, tj);
+  if (!gram_block(P, e, ti, tj)) return;   // block-uniform
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r0 = ti * kHoTile + (wave >> 1) * 48;
-  const int c0 = tj * kHoTile + (wave & 1) * 48;
+  const int rows = P.ntiles * kHoTile;
+  const int lr = lane & 15, lk = lane >> 4;
+  const size_t eo = (size_t)e * P.qp * rows;
+  const size_t oa = eo + (size_t)(ti * kHoTile + (wave >> 1) * 48 + lr) * 16 + 4 * lk;
+  const size_t ob = eo + (size_t)(tj * kHoTile + (wave & 1) * 48 + lr) * 16 + 4 * lk;
+  const s4* Ah = reinterpret_cast<const s4*>(P.Zh + oa);
+  const s4* Al = reinterpret_cast<const s4*>(P.Zl + oa);
+  const s4* Bh = reinterpret_cast<const s4*>(P.Zh + ob);
+  const s4* Bl = reinterpret_cast<const s4*>(P.Zl + ob);
+  const size_t kstride = (size_t)rows * 4;   // in s4 units (16 shorts per row per k-block)
+  const int nk = P.qp / 16;
   f4 acc[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int rows = P.ntiles * kHoTile;
-  const size_t estride = (size_t)P.qp * rows;
-  const short* Hh = P.Zh + (size_t)e * estride;
-  const short* Hl = P.Zl + (size_t)e * estride;
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int kb = 0; kb < P.qp / 16; ++kb) {
-    const size_t kofs = (size_t)kb * rows * 16;
-    s4 ah[3], al[3], bh[3], bl[3];
+  s4 ah[2][3], al[2][3], bh[2][3], bl[2][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const size_t oa = kofs + (size_t)(r0 + 16 * i + lr) * 16 + 4 * lk;
-      const size_t ob = kofs + (size_t)(c0 + 16 * i + lr) * 16 + 4 * lk;
-      ah[i] = *reinterpret_cast<const s4*>(Hh + oa);
-      al[i] = *reinterpret_cast<const s4*>(Hl + oa);
-      bh[i] = *reinterpret_cast<const s4*>(Hh + ob);
-      bl[i] = *reinterpret_cast<const s4*>(Hl + ob);
-    }
+  for (int i = 0; i < 3; ++i) {
+    ah[0][i] = Ah[64 * i]; al[0][i] = Al[64 * i];
+    bh[0][i] = Bh[64 * i]; bl[0][i] = Bl[64 * i];
+  }
+  for (int kb = 0; kb < nk; ++kb) {
+    // nk = qp/16 can be odd: the buffer index is kb & 1, resolved by unrolling two steps
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1) { if (++kb >= nk) break; }
+      const int cur = u, nxt = u ^ 1;
+      const size_t on = (size_t)(kb + 1 < nk ? kb + 1 : nk - 1) * kstride;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < 3; ++i) {
+        ah[nxt][i] = Ah[on + 64 * i]; al[nxt][i] = Al[on + 64 * i];
+        bh[nxt][i] = Bh[on + 64 * i]; bl[nxt][i] = Bl[on + 64 * i];
       }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[cur][i], bh[cur][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[cur][i], bl[cur][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[cur][i], bh[cur][j], acc[i][j], 0, 0, 0);
+        }
+    }
   }
   // bf16 16x16x16 output layout: lane l, reg r -> row 4(l/16) + r, col l%16
-  const int tr0 = (wave >> 1) * 48, tc0 = (wave & 1) * 48;
+  const int tc0 = (wave & 1) * 48;
+  for (int h = 0; h < 2; ++h) {
+    if ((wave >> 1) == h) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Gt[(tr0 + 16 * i + 4 * lk + r) * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
-  __syncthreads();
-  gram_epilogue(P, Gt, e, ti, tj);
+          for (int r = 0; r < 4; ++r)
+            Gt[(16 * i + 4 * lk + r) * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
+    }
+    __syncthreads();
+    gram_epilogue(P, Gt, e, ti, tj, h);
+    __syncthreads();
+  }
 }
 
 // One thread per node-pair block p: sum its staged element blocks in ascending
@@ -408,6 +455,7 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.E = m->E;
   P.npe = npe;
   P.ntiles = ntiles;
+  P.npairs = ntiles * (ntiles + 1) / 2;
   P.qp = qp;
   P.Z = Z;
   P.Zh = Zh;
@@ -417,10 +465,12 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.iso = C81 == nullptr;
   if (C81) std::memcpy(P.C, C81, sizeof(P.C));
   P.S = S;
-  const size_t lds = sizeof(double) * kHoTile * kHoTile;
+  const size_t lds = sizeof(double) * 48 * kHoTile;
   auto fn = bf ? k_gram_bf16x3 : k_gram_f64;
   LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const dim3 grid((unsigned)m->E, (unsigned)(ntiles * (ntiles + 1) / 2));
+  const int64_t nblocks = (m->E * P.npairs + kHoXcds - 1) / kHoXcds * kHoXcds;
+  if (nblocks >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "high-order path: too many elements");
+  const dim3 grid((unsigned)nblocks);
   LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
   hipLaunchKernelGGL(fn, grid, dim3(256), lds, st, P);
   LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
