@@ -165,6 +165,44 @@ LLFEM_API int llfem_solve_csc(llfem_ctx* ctx, int64_t ndof, const int64_t* colpt
                     const int64_t* fixed, int64_t nfixed, const double* u_fixed, int32_t method,
                     double reltol, int64_t maxiter, double* u_out, llfem_stats* stats);
 
+/* ---- batched assembly ----------------------------------------------------
+ * Replaces a host loop of stiffnessMatrix(problem) / massMatrix(problem;
+ * lumped) (src/linear.jl:70-87, 1843-2004) over independent problems: all
+ * (mesh, operator) pairs are integrated by ONE persistent panel launch
+ * (wavefronts take panel work from a device queue), after one batched
+ * geometry launch.  meshes[i] / descs[i] for i < n; a mesh may appear twice
+ * (stiffness and mass).  All meshes must share one context.  Values stay on
+ * the device (llfem_mesh_values copies them out).  Pairs the persistent
+ * kernel does not cover (general D, FP32/BF16 variants, npe not in
+ * {4,6,8,10}, two-byte element->nnz maps, order >= 3 hexahedra) are
+ * assembled by llfem_assemble inside the call.  stats (optional) sums over
+ * the batch: kernel_ms = geometry + panel device time, main_kernel_ms = the
+ * persistent launch, alg_bytes = sum of 24N + 4 n E + 8 nnz per pair. */
+LLFEM_API int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const llfem_op_desc* descs,
+                                   llfem_stats* stats);
+
+/* Copy the device-resident values of the last assembly of `op` on `mesh`
+ * (order of llfem_mesh_pattern(op); lumped=1: the pdim*N diagonal). */
+LLFEM_API int llfem_mesh_values(llfem_mesh* mesh, int32_t op, int32_t lumped, double* nzval_out);
+
+/* ---- autotuning ------------------------------------------------------------
+ * Times the panel assembly of `desc` on `sample` over a grid of launch
+ * shapes (launch-width multiplier of the record planner x waves per block),
+ * keeps the fastest for the sample's element type in the context (meshes
+ * created afterwards, and llfem_assemble / llfem_assemble_batch, use it) and
+ * re-plans the sample.  No reference counterpart: the reference's
+ * per-element loops have no launch shape. */
+typedef struct {
+  int32_t elem_type;       /* gmsh element type code tuned */
+  int32_t mult;            /* chosen launch-width multiplier */
+  int32_t waves_per_block; /* chosen waves per block */
+  int32_t candidates;      /* shapes timed */
+  double best_ms;          /* device time of the chosen shape */
+  double default_ms;       /* device time of the default shape */
+} llfem_tune_result;
+
+LLFEM_API int llfem_autotune(llfem_mesh* sample, const llfem_op_desc* desc, llfem_tune_result* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,465 +33,11 @@
 
 #include "common.h"
 
+#include "panel.h"
+
 namespace llfem {
 
 namespace {
-
-constexpr int kMaxQ = 1024;
-constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per CU / max per workgroup
-constexpr int kXcds = 8;                    // MI355X accelerator complex dies
-
-// ---------------------------------------------------------------------------
-// geometry
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double inv3(const double* J, double* Mi) {
-  // cofactor inverse, as inv3x3_fast (src/linear.jl:89-114); J row-major
-  // J[i][al] = dx_i/dxi_al; Mi row-major = J^{-1}, Mi[al][i] = dxi_al/dx_i.
-  double c11 = J[4] * J[8] - J[5] * J[7];
-  double c12 = -(J[3] * J[8] - J[5] * J[6]);
-  double c13 = J[3] * J[7] - J[4] * J[6];
-  double c21 = -(J[1] * J[8] - J[2] * J[7]);
-  double c22 = J[0] * J[8] - J[2] * J[6];
-  double c23 = -(J[0] * J[7] - J[1] * J[6]);
-  double c31 = J[1] * J[5] - J[2] * J[4];
-  double c32 = -(J[0] * J[5] - J[2] * J[3]);
-  double c33 = J[0] * J[4] - J[1] * J[3];
-  double det = J[0] * c11 + J[1] * c12 + J[2] * c13;
-  double id = 1.0 / det;
-  Mi[0] = c11 * id; Mi[1] = c21 * id; Mi[2] = c31 * id;
-  Mi[3] = c12 * id; Mi[4] = c22 * id; Mi[5] = c32 * id;
-  Mi[6] = c13 * id; Mi[7] = c23 * id; Mi[8] = c33 * id;
-  return det;
-}
-
-// J[i][al] = sum_a x_i^a dN_a[al], evaluated on coordinates relative to the
-// element's first node (sum_a dN_a = 0 makes this the same Jacobian, without
-// the cancellation of large absolute coordinates).
-__device__ __forceinline__ void jacobian(int64_t E, int64_t N, int npe, int64_t e,
-                                         const double* __restrict__ xyz,
-                                         const int32_t* __restrict__ conn,
-                                         const double* __restrict__ dNq, double* J) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) J[k] = 0.0;
-  const int32_t v0 = conn[e];
-  const double x0 = xyz[v0], y0 = xyz[N + v0], z0 = xyz[2 * N + v0];
-  for (int a = 1; a < npe; ++a) {
-    const int32_t v = conn[(int64_t)a * E + e];
-    const double x = xyz[v] - x0, y = xyz[N + v] - y0, z = xyz[2 * N + v] - z0;
-    const double g0 = dNq[3 * a], g1 = dNq[3 * a + 1], g2 = dNq[3 * a + 2];
-    J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
-    J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
-    J[6] += z * g0; J[7] += z * g1; J[8] += z * g2;
-  }
-}
-
-// flags |= 1 if some element's Jacobian varies over the points, |= 2 if some
-// |det J| is zero or not finite (the reference's singular-geometry error).
-__global__ void k_geom_check(int64_t E, int64_t N, int npe, int q, const double* __restrict__ xyz,
-                             const int32_t* __restrict__ conn, const double* __restrict__ dN,
-                             int* __restrict__ flags) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  double J0[9], J[9], Mi[9];
-  jacobian(E, N, npe, e, xyz, conn, dN, J0);
-  double scale = 0.0;
-  for (int k = 0; k < 9; ++k) scale = fmax(scale, fabs(J0[k]));
-  int f = 0;
-  double det = inv3(J0, Mi);
-  if (!(fabs(det) > 0.0) || !isfinite(det)) f |= 2;
-  for (int p = 1; p < q; ++p) {
-    jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
-    double diff = 0.0;
-    for (int k = 0; k < 9; ++k) diff = fmax(diff, fabs(J[k] - J0[k]));
-    if (diff > 1e-12 * scale) f |= 1;
-    double d = inv3(J, Mi);
-    if (!(fabs(d) > 0.0) || !isfinite(d)) f |= 2;
-  }
-  if (f) atomicOr(flags, f);
-}
-
-// Geometry records, one 128-byte line each (element-major, so a gather by
-// element id touches one cache line):
-//   double[0..8] Minv (row-major J^{-1}), double[9] wd,
-//   float[20..29] the same ten values in FP32 (for the FP32 integration path).
-// Affine: one record per element, wd = |det J|.  Otherwise one record per
-// (element, point), wd = |det J_p| w_p.
-constexpr int kGeoStride = 16;   // doubles
-
-__global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
-                           const double* __restrict__ xyz, const int32_t* __restrict__ conn,
-                           const double* __restrict__ dN, const double* __restrict__ w,
-                           double* __restrict__ geo) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  double J[9], Mi[9];
-  const int np = affine ? 1 : q;
-  for (int p = 0; p < np; ++p) {
-    jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
-    const double det = inv3(J, Mi);
-    const double wd = affine ? fabs(det) : fabs(det) * w[p];
-    double* g = geo + ((size_t)e * np + p) * kGeoStride;
-    double2* g2 = reinterpret_cast<double2*>(g);
-    g2[0] = make_double2(Mi[0], Mi[1]);
-    g2[1] = make_double2(Mi[2], Mi[3]);
-    g2[2] = make_double2(Mi[4], Mi[5]);
-    g2[3] = make_double2(Mi[6], Mi[7]);
-    g2[4] = make_double2(Mi[8], wd);
-    float4* f4 = reinterpret_cast<float4*>(g + 10);
-    f4[0] = make_float4((float)Mi[0], (float)Mi[1], (float)Mi[2], (float)Mi[3]);
-    f4[1] = make_float4((float)Mi[4], (float)Mi[5], (float)Mi[6], (float)Mi[7]);
-    f4[2] = make_float4((float)Mi[8], (float)wd, 0.f, 0.f);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// node-panel kernel
-// ---------------------------------------------------------------------------
-struct PanelParams {
-  const void* rec;     // step records of the launch
-  int rec_bytes;
-  const int2* desc;    // node descriptors {adj offset, R} of the launch
-  int64_t waves;
-  int rounds, cnt, max_r;
-  const double* geo;
-  int64_t E;
-  int q;
-  const double* tab;   // double table, see llfem_assemble
-  const float* tabf;   // FP32 copy of the same table
-  double lam, mu, rho;
-  double* out;
-  int lds_per_wave;    // doubles
-  double C[81];        // general constitutive tensor C[d][al][c][be]
-};
-
-// Panel update: plain read-add-write.  No two lanes of a step address the
-// same live row; successive steps stay ordered because LDS executes one wave's
-// instructions in order and a compiler barrier separates the steps.
-__device__ __forceinline__ void lds_add(double* p, double v) { *p += v; }
-
-template <typename Real>
-struct Geo {
-  Real Mi[9];
-  Real wd;
-};
-
-template <typename Real>
-__device__ __forceinline__ Geo<Real> load_geo(const double* __restrict__ rec) {
-  Geo<Real> g;
-  if constexpr (sizeof(Real) == 8) {
-    const double2* r2 = reinterpret_cast<const double2*>(rec);
-    double2 v0 = r2[0], v1 = r2[1], v2 = r2[2], v3 = r2[3], v4 = r2[4];
-    g.Mi[0] = v0.x; g.Mi[1] = v0.y; g.Mi[2] = v1.x; g.Mi[3] = v1.y; g.Mi[4] = v2.x;
-    g.Mi[5] = v2.y; g.Mi[6] = v3.x; g.Mi[7] = v3.y; g.Mi[8] = v4.x; g.wd = v4.y;
-  } else {
-    const float4* r4 = reinterpret_cast<const float4*>(rec + 10);
-    float4 v0 = r4[0], v1 = r4[1];
-    float2 v2 = reinterpret_cast<const float2*>(rec + 10)[4];
-    g.Mi[0] = v0.x; g.Mi[1] = v0.y; g.Mi[2] = v0.z; g.Mi[3] = v0.w; g.Mi[4] = v1.x;
-    g.Mi[5] = v1.y; g.Mi[6] = v1.z; g.Mi[7] = v1.w; g.Mi[8] = v2.x; g.wd = v2.y;
-  }
-  return g;
-}
-
-template <typename Real>
-__device__ __forceinline__ void load9(const double* d, const float* f, Real* o) {
-  if constexpr (sizeof(Real) == 8) {
-#pragma unroll
-    for (int m = 0; m < 9; ++m) o[m] = d[m];
-  } else {
-#pragma unroll
-    for (int m = 0; m < 9; ++m) o[m] = f[m];
-  }
-}
-
-// G = Minv^T Gh Minv for one block (affine elements).
-template <typename Real>
-__device__ __forceinline__ void block_G_affine(const Geo<Real>& g, const Real* Gh, Real* G) {
-  const Real* Mi = g.Mi;
-  Real T[9];   // T[al][k] = sum_be Gh[al][be] Mi[be][k]
-#pragma unroll
-  for (int al = 0; al < 3; ++al)
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk)
-      T[al * 3 + kk] = Gh[al * 3] * Mi[kk] + Gh[al * 3 + 1] * Mi[3 + kk] + Gh[al * 3 + 2] * Mi[6 + kk];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk)
-      G[i * 3 + kk] = Mi[i] * T[kk] + Mi[3 + i] * T[3 + kk] + Mi[6 + i] * T[6 + kk];
-}
-
-// K[d][c] = Ke[(b,d),(a,c)] from G and the material.
-template <bool ISO, typename Real>
-__device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Real scale, Real* K) {
-  if constexpr (ISO) {
-    const Real l = (Real)P.lam * scale, u = (Real)P.mu * scale;
-    const Real tr = u * (G[0] + G[4] + G[8]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d];
-    K[0] += tr;
-    K[4] += tr;
-    K[8] += tr;
-  } else {
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        Real acc = 0;
-#pragma unroll
-        for (int al = 0; al < 3; ++al)
-#pragma unroll
-          for (int be = 0; be < 3; ++be)
-            acc += (Real)P.C[((d * 3 + al) * 3 + c) * 3 + be] * G[al * 3 + be];
-        K[d * 3 + c] = acc * scale;
-      }
-  }
-}
-
-template <typename Real>
-struct TabBlk {
-  Real v[9];
-};
-
-// Operator-table entry of (a, b): Ghat (9 values) for stiffness, Mhat for mass.
-template <int NPE, int OP, bool AFFINE, typename Real>
-__device__ __forceinline__ TabBlk<Real> load_tab(const PanelParams& P, int a, int b) {
-  TabBlk<Real> t;
-  if constexpr (AFFINE && OP == LLFEM_OP_STIFFNESS) {
-    load9<Real>(P.tab + (size_t)(a * NPE + b) * 9, P.tabf + (size_t)(a * NPE + b) * 9, t.v);
-  } else if constexpr (AFFINE) {
-    t.v[0] = (Real)P.tab[a * NPE + b];
-  }
-  return t;
-}
-
-// Add element block Ke[(b,:),(a,:)] to the panel rows of slot s (or, for a
-// non-affine element, integrate it point by point first).
-template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
-__device__ __forceinline__ void contribute(const PanelParams& P, const TabBlk<Real>& tb,
-                                           double* __restrict__ wl, int base, int R, int b, int e,
-                                           int a, int s, const Geo<Real>& g) {
-  if constexpr (OP == LLFEM_OP_STIFFNESS) {
-    Real G[9];
-    Real scale;
-    if constexpr (AFFINE) {
-      block_G_affine<Real>(g, tb.v, G);
-      scale = g.wd;
-    } else {
-#pragma unroll
-      for (int m = 0; m < 9; ++m) G[m] = 0;
-      scale = 1;
-      for (int p = 0; p < P.q; ++p) {
-        const Geo<Real> gp = load_geo<Real>(P.geo + ((size_t)e * P.q + p) * kGeoStride);
-        const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
-        const double* ha = P.tab + ((size_t)p * NPE + a) * 3;
-        Real gb[3], ga[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          gb[i] = gp.Mi[i] * (Real)hb[0] + gp.Mi[3 + i] * (Real)hb[1] + gp.Mi[6 + i] * (Real)hb[2];
-          ga[i] = (gp.Mi[i] * (Real)ha[0] + gp.Mi[3 + i] * (Real)ha[1] + gp.Mi[6 + i] * (Real)ha[2]) * gp.wd;
-        }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) G[i * 3 + kk] += gb[i] * ga[kk];
-      }
-    }
-    Real K[9];
-    block_K<ISO, Real>(P, G, scale, K);
-    double* dst = wl + base + 3 * s;
-    double old[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) old[c * 3 + d] = dst[c * 3 * R + d];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) dst[c * 3 * R + d] = old[c * 3 + d] + (double)K[d * 3 + c];
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    double mv;
-    if constexpr (AFFINE) {
-      mv = P.rho * (double)g.wd * (double)tb.v[0];
-    } else {
-      mv = 0.0;
-      for (int p = 0; p < P.q; ++p)
-        mv += P.geo[((size_t)e * P.q + p) * kGeoStride + 9] * P.tab[p * NPE + b] * P.tab[p * NPE + a];
-      mv *= P.rho;
-    }
-    lds_add(wl + base + s, mv);
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
-// Node-panel kernel over processing-order step records.
-//
-// A wavefront owns NPW = 64/NPE lane groups; lane (n, b) handles element node
-// b.  Lane group n streams `rounds` nodes one after another: in round i it
-// accumulates node p = (w*G + i)*NPW + n's panel from `cnt` step records
-// (one per incidence, null-padded), then writes the finished panel to HBM
-// and clears it.  Records of the NPW groups of one step are adjacent, so each
-// step is one coalesced 16-byte load per lane -- no pointer chasing -- and the
-// prefetch ring runs straight through round boundaries:
-//   rec[v]  step record, fetched 2*PF steps ahead;
-//   g[v]    element geometry, fetched PF steps ahead;
-//   tb[v]   reference-table block Ghat[a][b], fetched PF steps ahead.
-// All ring slot indices are compile-time (the loop is unrolled by 2*PF) and
-// every load is unconditional (clamped into the wave's record range), so the
-// waits are exact.  The only LDS traffic in a step is the panel update: a
-// plain read-add-write (volatile), conflict-free within a step because the
-// lanes of a group address distinct rows and LDS executes one wave's
-// instructions in order -- sums are deterministic and need no atomics.
-constexpr int PF = 2;
-
-struct StepRec {
-  int e, a, s;   // element, local index (-1: null step), panel row of node b
-};
-
-template <int NPE, typename MapT>
-__device__ __forceinline__ StepRec load_rec(const PanelParams& P, int64_t r, int b) {
-  StepRec o;
-  if constexpr (NPE <= 11 && sizeof(MapT) == 1) {
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(P.rec) + r);
-    o.e = (int)v.x;
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-    const int byte = 5 + b;
-    const uint32_t a8 = v.y & 0xffu;
-    o.a = a8 == 0xffu ? -1 : (int)a8;
-    o.s = (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
-  } else {
-    const uint8_t* base = static_cast<const uint8_t*>(P.rec) + r * P.rec_bytes;
-    const int2 h = *reinterpret_cast<const int2*>(base);
-    o.e = h.x;
-    o.a = (int)(int16_t)(h.y & 0xffff);
-    o.s = (int)reinterpret_cast<const MapT*>(base + 8)[b];
-  }
-  return o;
-}
-
-template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
-__global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
-  extern __shared__ double lds[];
-  constexpr int NPW = 64 / NPE;
-  constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  // XCD-aware work order: the dispatcher deals blocks round-robin to the 8
-  // XCDs, so block b runs on XCD b % 8.  Give XCD x the contiguous block range
-  // [x*per, (x+1)*per) of the processing order; neighbouring node groups then
-  // share an L2 and re-read element geometry from it rather than from HBM.
-  // The host pads the grid to a multiple of kXcds, which makes the remap a
-  // bijection of [0, gridDim.x); padding blocks fail the w < waves test.
-  const int64_t per = gridDim.x / kXcds;
-  const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;   // logical block
-  const int64_t w = lb * (blockDim.x >> 6) + wave;
-  if (w >= P.waves) return;                       // no block-level barriers below
-  double* wl = lds + (size_t)wave * P.lds_per_wave;
-  double* const dummy = wl + (size_t)NPW * P.max_r * BLK;   // 9 doubles
-  const int n = lane / NPE;
-  const int b = lane - n * NPE;
-  const bool grp = n < NPW;
-  const int G = P.rounds, cnt = P.cnt;
-  const int64_t nsteps = (int64_t)G * cnt;
-  const int64_t rbase = w * nsteps * NPW;         // first record of this wave
-  const int64_t rlast = rbase + nsteps * NPW - 1;
-  const int64_t lane_rec = grp ? n : 0;
-  double* const pan = wl + (size_t)(grp ? n : 0) * P.max_r * BLK;
-
-  for (int i = lane; i < NPW * P.max_r * BLK + 9; i += 64) wl[i] = 0.0;
-
-  // node descriptors of the current and next round
-  const int2* desc = P.desc + w * (int64_t)G * NPW;
-  int2 dcur = desc[lane_rec];
-  int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
-
-  StepRec rec[2 * PF];
-  Geo<Real> g[PF];
-  TabBlk<Real> tb[PF];
-#pragma unroll
-  for (int v = 0; v < 2 * PF; ++v)
-    rec[v] = load_rec<NPE, MapT>(P, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
-#pragma unroll
-  for (int v = 0; v < PF; ++v) {
-    g[v] = load_geo<Real>(P.geo + (size_t)rec[v].e * kGeoStride);
-    tb[v] = load_tab<NPE, OP, AFFINE, Real>(P, rec[v].a < 0 ? 0 : rec[v].a, b);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-
-  int t = 0, round = 0;
-  for (int64_t s0 = 0; s0 < nsteps; s0 += 2 * PF) {
-#pragma unroll
-    for (int u = 0; u < 2 * PF; ++u) {
-      const int64_t s = s0 + u;
-      if (s < nsteps) {
-        const StepRec& cr = rec[u];
-        const bool live = grp && cr.a >= 0 && dcur.y > 0;
-        contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, tb[u % PF], live ? pan : dummy,
-                                                     0, live ? dcur.y : 0, b, cr.e,
-                                                     cr.a < 0 ? 0 : cr.a, live ? cr.s : 0,
-                                                     g[u % PF]);
-        if (++t == cnt) {   // round complete: write the NPW panels, clear them
-          t = 0;
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-#pragma unroll
-          for (int m = 0; m < NPW; ++m) {
-            const int Rm = __shfl(dcur.y, m * NPE);
-            const int64_t om = (int64_t)__shfl(dcur.x, m * NPE);
-            double* pm = wl + (size_t)m * P.max_r * BLK;
-            if constexpr (OP == LLFEM_OP_STIFFNESS) {
-              double* o = P.out + 9 * om;
-              for (int i = lane; i < 9 * Rm; i += 64) {
-                __builtin_nontemporal_store(pm[i], o + i);
-                pm[i] = 0.0;
-              }
-            } else {
-              double* o = P.out + 3 * om;
-              for (int i = lane; i < 3 * Rm; i += 64) {
-                const int c = i / Rm;
-                __builtin_nontemporal_store(pm[i - c * Rm], o + i);
-              }
-            }
-          }
-          if constexpr (OP != LLFEM_OP_STIFFNESS) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            for (int i = lane; i < NPW * P.max_r; i += 64) wl[i] = 0.0;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          ++round;
-          dcur = dnext;
-          dnext = desc[(int64_t)min(round + 1, G - 1) * NPW + lane_rec];
-        }
-      }
-      // ring refill: geometry/table for step s+PF, record for step s+2PF
-      const StepRec& nx = rec[(u + PF) % (2 * PF)];
-      g[u % PF] = load_geo<Real>(P.geo + (size_t)nx.e * kGeoStride);
-      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(P, nx.a < 0 ? 0 : nx.a, b);
-      rec[u] = load_rec<NPE, MapT>(P, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
-    }
-  }
-}
-
-// lumped mass: row sums of the consistent mass panels (src/linear.jl:1999-2001)
-__global__ void k_lump(int64_t N, const int64_t* __restrict__ adj_ptr, const double* __restrict__ M,
-                       double* __restrict__ out) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  const int64_t a0 = adj_ptr[j];
-  const int R = (int)(adj_ptr[j + 1] - a0);
-  for (int c = 0; c < 3; ++c) {
-    double s = 0.0;
-    for (int i = 0; i < R; ++i) s += M[3 * a0 + (int64_t)c * R + i];
-    out[3 * j + c] = s;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // dispatch
@@ -531,16 +77,6 @@ PanelFn select_panel(int npe, int map_bytes, int op, bool affine, bool iso, bool
   }
 }
 
-// Voigt index of the symmetric pair {i,j} in SymGrad row order
-// [xx,yy,zz,xy,xz,yz] (src/multifield.jl:635-657).
-int voigt(int i, int j) {
-  if (i == j) return i;
-  int a = i < j ? i : j, b = i < j ? j : i;
-  if (a == 0 && b == 1) return 3;
-  if (a == 0 && b == 2) return 4;
-  return 5;
-}
-
 int ensure_buf(llfem_ctx* ctx, double** p, size_t* cur, size_t bytes) {
   if (*p && *cur >= bytes) return LLFEM_OK;
   if (*p) dfree(ctx, *p, *cur);
@@ -552,6 +88,55 @@ int ensure_buf(llfem_ctx* ctx, double** p, size_t* cur, size_t bytes) {
 }
 
 }  // namespace
+
+// Voigt index of the symmetric pair {i,j} in SymGrad row order
+// [xx,yy,zz,xy,xz,yz] (src/multifield.jl:635-657).
+int voigt(int i, int j) {
+  if (i == j) return i;
+  int a = i < j ? i : j, b = i < j ? j : i;
+  if (a == 0 && b == 1) return 3;
+  if (a == 0 && b == 2) return 4;
+  return 5;
+}
+
+// Reference operator table of a panel launch.  Stiffness: affine -> Ghat
+// [a][b][al][be] = sum_q w_q dN_b[al] dN_a[be]; curved -> dN itself.  Mass:
+// affine -> Mhat[b][a] = sum_q w_q N_b N_a; curved -> N itself.
+std::vector<double> operator_table(const llfem_op_desc* d, int npe, bool affine) {
+  const int q = d->q;
+  std::vector<double> tab;
+  if (d->op == LLFEM_OP_STIFFNESS) {
+    if (affine) {
+      tab.assign((size_t)npe * npe * 9, 0.0);
+      for (int b = 0; b < npe; ++b)
+        for (int a = 0; a < npe; ++a)
+          for (int al = 0; al < 3; ++al)
+            for (int be = 0; be < 3; ++be) {
+              double s = 0.0;
+              for (int p = 0; p < q; ++p)
+                s += d->weights[p] * d->dN_ref[((size_t)p * npe + b) * 3 + al] *
+                     d->dN_ref[((size_t)p * npe + a) * 3 + be];
+              tab[((size_t)a * npe + b) * 9 + al * 3 + be] = s;   // [a][b] order
+            }
+    } else {
+      tab.assign(d->dN_ref, d->dN_ref + (size_t)q * npe * 3);
+    }
+  } else {
+    if (affine) {
+      tab.assign((size_t)npe * npe, 0.0);
+      for (int b = 0; b < npe; ++b)
+        for (int a = 0; a < npe; ++a) {
+          double s = 0.0;
+          for (int p = 0; p < q; ++p)
+            s += d->weights[p] * d->N_ref[(size_t)p * npe + b] * d->N_ref[(size_t)p * npe + a];
+          tab[(size_t)b * npe + a] = s;
+        }
+    } else {
+      tab.assign(d->N_ref, d->N_ref + (size_t)q * npe);
+    }
+  }
+  return tab;
+}
 
 // Upload [dN | w | op-table space] and run the geometry check once per rule.
 int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
@@ -640,40 +225,9 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   if (!fn && !highorder)
     return fail(LLFEM_EUNSUP, "llfem_assemble: no panel kernel for npe=" + std::to_string(npe));
 
-  // per-operator reference tables (host, FP64)
+  // per-operator reference tables (host, FP64); the Gram kernels read dN and w directly
   std::vector<double> tab;
-  if (highorder) {
-    // the Gram kernels read dN and w directly
-  } else if (op == LLFEM_OP_STIFFNESS) {
-    if (affine) {
-      tab.assign((size_t)npe * npe * 9, 0.0);
-      for (int b = 0; b < npe; ++b)
-        for (int a = 0; a < npe; ++a)
-          for (int al = 0; al < 3; ++al)
-            for (int be = 0; be < 3; ++be) {
-              double s = 0.0;
-              for (int p = 0; p < q; ++p)
-                s += d->weights[p] * d->dN_ref[((size_t)p * npe + b) * 3 + al] *
-                     d->dN_ref[((size_t)p * npe + a) * 3 + be];
-              tab[((size_t)a * npe + b) * 9 + al * 3 + be] = s;   // [a][b] order
-            }
-    } else {
-      tab.assign(d->dN_ref, d->dN_ref + (size_t)q * npe * 3);
-    }
-  } else {
-    if (affine) {
-      tab.assign((size_t)npe * npe, 0.0);
-      for (int b = 0; b < npe; ++b)
-        for (int a = 0; a < npe; ++a) {
-          double s = 0.0;
-          for (int p = 0; p < q; ++p)
-            s += d->weights[p] * d->N_ref[(size_t)p * npe + b] * d->N_ref[(size_t)p * npe + a];
-          tab[(size_t)b * npe + a] = s;
-        }
-    } else {
-      tab.assign(d->N_ref, d->N_ref + (size_t)q * npe);
-    }
-  }
+  if (!highorder) tab = operator_table(d, npe, affine);
   const size_t off = (size_t)q * npe * 3 + q;
   std::vector<float> tabf(tab.begin(), tab.end());
   float* d_tabf = reinterpret_cast<float*>(m->d_tab + off + tab.size());
@@ -736,6 +290,10 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
       if (resident > best) { best = resident; wpb = (int)c; }
     }
     if (wpb < 1) return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
+    const auto tu = ctx->tune.find(m->etype);
+    if (tu != ctx->tune.end() && tu->second.wpb > 0 && (size_t)tu->second.wpb <= max_wpb &&
+        (size_t)tu->second.wpb * wave_bytes <= kLdsPerCU)
+      wpb = tu->second.wpb;
     const int64_t used = (L.waves + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
     const size_t lds_bytes = wave_bytes * wpb;
@@ -754,6 +312,9 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   if (op == LLFEM_OP_STIFFNESS) {
     m->have_K = true;
     m->solver_ready = false;
+  } else {
+    m->have_M = true;
+    m->have_lumped = d->lumped != 0;
   }
   if (nzval_out) {
     if (op == LLFEM_OP_MASS && d->lumped)

@@ -58,6 +58,21 @@ struct RuleState {
   bool singular = false;
 };
 
+// Tuned panel parameters of one element type (llfem_autotune): launch width
+// multiplier of the record planner (0 = default) and waves per block (0 =
+// the most resident waves).
+struct Tune {
+  int32_t mult = 0;
+  int32_t wpb = 0;
+  double ms = 0.0;
+};
+
+// A device copy of a reference table, shared by every mesh of a context.
+struct DevTable {
+  double* ptr = nullptr;
+  size_t bytes = 0;
+};
+
 }  // namespace llfem
 
 struct llfem_ctx {
@@ -67,6 +82,10 @@ struct llfem_ctx {
   size_t in_use = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int num_cus = 0;
+  std::map<int, llfem::Tune> tune;                 // by gmsh element type
+  std::map<uint64_t, llfem::DevTable> tables;      // by content hash
+  void* d_scratch = nullptr;                       // batch descriptors + queue counter
+  size_t scratch_bytes = 0;
 };
 
 struct llfem_mesh {
@@ -109,6 +128,8 @@ struct llfem_mesh {
   double* d_valM = nullptr;
   double* d_lumped = nullptr;
   bool have_K = false;
+  bool have_M = false;
+  bool have_lumped = false;
   // solver state cached per assembled K
   float* d_valK32 = nullptr;
   double* d_diagK = nullptr;
@@ -139,6 +160,7 @@ void drelease(llfem_ctx* ctx, T*& p, size_t count) {
 }
 
 int build_pattern(llfem_mesh* m);
+int rebuild_records(llfem_mesh* m);
 
 // Element-type facts the kernels need.
 inline int etype_dim(int et) {
@@ -148,6 +170,9 @@ inline int etype_dim(int et) {
     default: return 0;
   }
 }
+
+std::vector<double> operator_table(const llfem_op_desc* d, int npe, bool affine);
+int voigt(int i, int j);
 
 int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const double* d_dN,
                        const double* d_w, double lam, double mu, const double* C81, double* out);

@@ -84,6 +84,9 @@ int llfem_ctx_destroy(llfem_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& kv : ctx->tables) llfem::dfree(ctx, kv.second.ptr, kv.second.bytes);
+  ctx->tables.clear();
+  if (ctx->d_scratch) llfem::dfree(ctx, ctx->d_scratch, ctx->scratch_bytes);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LLFEM_OK;

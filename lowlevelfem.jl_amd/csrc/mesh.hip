@@ -232,7 +232,10 @@ int build_records(llfem_mesh* m, const std::vector<int32_t>& h_inc_ptr) {
     const int blk = 9;   // LDS sized for stiffness; mass uses less
     const int64_t wave_bytes = 8 * ((int64_t)npw * rmax * blk + 9);
     const int64_t wpc = std::max<int64_t>(1, std::min<int64_t>(16, (160 * 1024) / wave_bytes));
-    const int64_t target = 8 * ncu * wpc;                       // wavefronts per launch
+    // wavefronts per launch: mult x resident waves (8 unless tuned)
+    const auto tu = ctx->tune.find(m->etype);
+    const int64_t mult = (tu != ctx->tune.end() && tu->second.mult > 0) ? tu->second.mult : 8;
+    const int64_t target = mult * ncu * wpc;
     int64_t G = (nodes + npw * target - 1) / (npw * target);
     G = std::max<int64_t>(1, std::min<int64_t>(G, 256));
     const int64_t per_wave = npw * G;
@@ -281,6 +284,22 @@ int build_records(llfem_mesh* m, const std::vector<int32_t>& h_inc_ptr) {
   LLFEM_HIP(hipStreamSynchronize(st));
   drelease(ctx, d_order, order.size() + 1);
   return LLFEM_OK;
+}
+
+// Re-plan the panel launches (after a tuning change); the pattern and the
+// element->nnz map are kept.
+int rebuild_records(llfem_mesh* m) {
+  llfem_ctx* ctx = m->ctx;
+  if (m->npe > 64) return LLFEM_OK;
+  std::vector<int32_t> h_inc_ptr(m->N + 1);
+  LLFEM_HIP(hipMemcpyAsync(h_inc_ptr.data(), m->d_inc_ptr, sizeof(int32_t) * (m->N + 1),
+                           hipMemcpyDeviceToHost, ctx->stream));
+  LLFEM_HIP(hipStreamSynchronize(ctx->stream));
+  if (m->d_rec) dfree(ctx, m->d_rec, (size_t)m->rec_count * m->rec_bytes);
+  m->d_rec = nullptr;
+  drelease(ctx, m->d_desc, m->desc_count);
+  m->rec_count = m->desc_count = 0;
+  return build_records(m, h_inc_ptr);
 }
 
 int build_pattern(llfem_mesh* m) {

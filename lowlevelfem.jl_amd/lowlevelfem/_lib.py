@@ -22,6 +22,7 @@ EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
     "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
     "llfem_assemble", "llfem_load_traction", "llfem_solve", "llfem_solve_csc",
+    "llfem_assemble_batch", "llfem_mesh_values", "llfem_autotune",
 ]
 
 
@@ -41,6 +42,11 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("elem_type", C.c_int32), ("mult", C.c_int32), ("waves_per_block", C.c_int32),
+                ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
 class LLFEMError(RuntimeError):
@@ -75,6 +81,9 @@ def load_library(path: str = LIB_PATH):
     lib.llfem_solve.argtypes = [P, P, P, I64, I32, P, I32, D, I64, P, C.POINTER(Stats)]
     lib.llfem_solve_csc.argtypes = [P, I64, P, P, P, I32, P, P, I64, P, I32, D, I64, P,
                                     C.POINTER(Stats)]
+    lib.llfem_assemble_batch.argtypes = [I32, P, P, C.POINTER(Stats)]
+    lib.llfem_mesh_values.argtypes = [P, I32, I32, P]
+    lib.llfem_autotune.argtypes = [P, C.POINTER(OpDesc), C.POINTER(TuneResult)]
     for name in EXPORTED:
         fn = getattr(lib, name)
         if name not in ("llfem_last_error", "llfem_ctx_bytes_in_use"):
@@ -131,6 +140,36 @@ def default_context() -> Context:
     return _default_ctx
 
 
+def make_desc(op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False, precision=FP64):
+    """OpDesc for llfem_assemble; returns (desc, arrays that must outlive it)."""
+    w = np.ascontiguousarray(tables.weights, dtype=np.float64)
+    Nr = np.ascontiguousarray(tables.N, dtype=np.float64)
+    dN = np.ascontiguousarray(tables.dN, dtype=np.float64)
+    Dm = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(6, 6)
+    desc = OpDesc(op, tables.q, ptr(w).value, ptr(Nr).value, ptr(dN).value,
+                  None if Dm is None else ptr(Dm).value, E, nu, rho, int(lumped), precision)
+    return desc, (w, Nr, dN, Dm)
+
+
+def assemble_batch(items):
+    """One persistent assembly launch over many (DeviceMesh, op, tables, kwargs)
+    items (llfem_assemble_batch); values stay on the device (DeviceMesh.values)."""
+    if not items:
+        return Stats()
+    ctx = items[0][0].ctx
+    n = len(items)
+    handles = (C.c_void_p * n)(*[it[0].h.value for it in items])
+    descs = (OpDesc * n)()
+    keep = []
+    for i, it in enumerate(items):
+        d, k = make_desc(it[1], it[2], **(it[3] if len(it) > 3 else {}))
+        descs[i] = d
+        keep.append(k)
+    st = Stats()
+    check(ctx.lib.llfem_assemble_batch(n, handles, descs, C.byref(st)))
+    return st
+
+
 class DeviceMesh:
     """Device-resident mesh + structural pattern (llfem_mesh)."""
 
@@ -158,12 +197,7 @@ class DeviceMesh:
 
     def assemble(self, op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False,
                  precision=FP64, copy_out=True):
-        w = np.ascontiguousarray(tables.weights, dtype=np.float64)
-        Nr = np.ascontiguousarray(tables.N, dtype=np.float64)
-        dN = np.ascontiguousarray(tables.dN, dtype=np.float64)
-        Dm = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(6, 6)
-        desc = OpDesc(op, tables.q, ptr(w).value, ptr(Nr).value, ptr(dN).value,
-                      None if Dm is None else ptr(Dm).value, E, nu, rho, int(lumped), precision)
+        desc, _keep = make_desc(op, tables, E, nu, rho, D, lumped, precision)
         st = Stats()
         out = None
         if copy_out:
@@ -171,6 +205,20 @@ class DeviceMesh:
             out = np.empty(n, dtype=np.float64)
         check(self.ctx.lib.llfem_assemble(self.h, C.byref(desc), ptr(out), C.byref(st)))
         return out, st
+
+    def values(self, op=OP_STIFFNESS, lumped=False):
+        """Device-resident values of the last assembly of `op` (llfem_mesh_values)."""
+        n = 3 * self.N if (op == OP_MASS and lumped) else self.pattern(op, with_arrays=False)
+        out = np.empty(n, dtype=np.float64)
+        check(self.ctx.lib.llfem_mesh_values(self.h, op, int(lumped), ptr(out)))
+        return out
+
+    def autotune(self, op, tables, **kw):
+        """Time launch shapes on this mesh and keep the best for its element type."""
+        desc, _keep = make_desc(op, tables, **kw)
+        res = TuneResult()
+        check(self.ctx.lib.llfem_autotune(self.h, C.byref(desc), C.byref(res)))
+        return res
 
     def traction(self, face_type, face_conn, tables, t):
         fc = np.ascontiguousarray(face_conn, dtype=np.int64)
