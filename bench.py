@@ -1,11 +1,18 @@
-"""Headline benchmark: FP64 stiffness assembly of the 40x40x160 quadratic-tet
-beam (Kuhn split, 1,536,000 tet10, 2,106,081 nodes, 6,318,243 DoF,
-535,898,889 structural non-zeros, 4-point rule, E = 2e5, nu = 0.3).
+"""Benchmarks of the assembly path.  Prints ONE JSON line per run.
 
-A step is one full assembly of K into the device-resident CSC values with
-the mesh, pattern and element->nnz map already in HBM.  Prints ONE JSON line.
+--workload tet10-beam (default, the headline): FP64 stiffness assembly of the
+  40x40x160 quadratic-tet beam (Kuhn split, 1,536,000 tet10, 2,106,081 nodes,
+  6,318,243 DoF, 535,898,889 structural non-zeros, 4-point rule, E = 2e5,
+  nu = 0.3).  A step is one full assembly of K into the device-resident CSC
+  values with the mesh, pattern and element->nnz map already in HBM.
+--workload hex125: FP64 stiffness of the 12^3 order-4 hexahedron cube (1,728
+  hex125, interior nodes perturbed by +-5% of h, 5x5x5 Gauss) on the matrix
+  cores, with the BF16x3 variant and its error against FP64 reported.
+--workload batch256: 256 seeded uneven problems (hex8 / tet10 / wedge6,
+  2k-400k DoF), stiffness + consistent mass of all of them in one persistent
+  launch per step, after autotuning the launch shape per element type.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload W]
 """
 import argparse
 import json
@@ -93,22 +100,7 @@ def calib_copy(nbytes):
     hip.hipFree(b)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cells", type=str, default=None, help="override nx,ny,nz (testing only)")
-    ap.add_argument("--calib-copy", type=int, default=0,
-                    help="after the steps, copy this many bytes device-to-device once "
-                         "(reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
-        sys.exit(2)
-
+def bench_tet10(args):
     from lowlevelfem import _lib, mesh as M, refelem as R
     cells = tuple(int(x) for x in args.cells.split(",")) if args.cells else CELLS
     t0 = time.perf_counter()
@@ -175,6 +167,266 @@ def main():
         calib_copy(args.calib_copy)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline((40, 40, 16) if not args.cells else cells)
+    return out
+
+
+FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_hex125(args):
+    """Order-4 hexahedra on the matrix cores (FP64 MFMA Gram product, and
+    BF16x3 with FP32 accumulation)."""
+    from lowlevelfem import _lib, mesh as M, refelem as R
+    n = int(args.cells) if args.cells else 12
+    m = M.structured_box_mesh(n=n, order=4, perturb=0.05, seed=12)
+    tab = R.rule_tables(m.etype, 9)
+    npe, q = m.npe, tab.q
+    ctx = _lib.default_context()
+    t0 = time.perf_counter()
+    dev = _lib.DeviceMesh(ctx, m.coords, m.etype, m.conn)
+    ctx.synchronize()
+    setup_s = time.perf_counter() - t0
+    nnz = dev.pattern(with_arrays=False)
+    log(f"hex125 {n}^3: {m.E} elements, {m.N} nodes, nnz {nnz}, setup {setup_s:.2f} s")
+
+    def run(precision):
+        for _ in range(args.warmup):
+            dev.assemble(_lib.OP_STIFFNESS, tab, copy_out=False, precision=precision)
+        ctx.synchronize()
+        ks, gs = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = dev.assemble(_lib.OP_STIFFNESS, tab, copy_out=False, precision=precision)
+            ks.append(st.kernel_ms)
+            gs.append(st.main_kernel_ms)
+        ctx.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / args.steps, float(np.mean(ks)), float(np.mean(gs))
+
+    ms64, k64, g64 = run(_lib.FP64)
+    ref = dev.values(_lib.OP_STIFFNESS)
+    ms16, k16, g16 = run(_lib.BF16X3)
+    v16 = dev.values(_lib.OP_STIFFNESS)
+    n3 = 3 * npe
+    ntiles = (n3 + 95) // 96
+    qp = (q + 15) // 16 * 16
+    f_ref = 2.0 * n3 * n3 * 6 * q               # dense B'DB per element, as the reference evaluates it
+    f_gram = n3 * (n3 + 1.0) * q                # symmetric Gram Z Z^T (upper triangle incl. diagonal)
+    f_exec = ntiles * (ntiles + 1) / 2 * 96 * 96 * 2.0 * qp   # MFMA flops issued (padded tiles)
+    achieved = f_gram * m.E / (g64 * 1e-3) / 1e12
+    out = {
+        "metric": "assembled elem/s (FP64 stiffness, order-4 hexahedra, Gram product on MFMA)",
+        "value": m.E / (ms64 * 1e-3),
+        "unit": "elem/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"hex125 unit cube {n}^3, interior nodes perturbed +-5% of h (seed 12), "
+                               "5x5x5 Gauss, FP64 stiffness into CSC",
+                   "elements": m.E, "nodes": m.N, "dofs": 3 * m.N, "nnz": nnz},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+        "flops_per_element": {"reference_dense_btdb": f_ref, "gram_useful": f_gram,
+                              "gram_mfma_issued": f_exec},
+        "kernel_ms": {"assembly_events": k64, "gram_kernel_events": g64},
+        "gram_mfma_issued_tflops": f_exec * m.E / (g64 * 1e-3) / 1e12,
+        "bf16x3": {"ms_per_step": ms16, "elem_per_s": m.E / (ms16 * 1e-3),
+                   "kernel_ms": {"assembly_events": k16, "gram_kernel_events": g16},
+                   "mfma_frac_of_bf16_peak": 3 * f_exec * m.E / (g16 * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+                   "rel_err_norm_vs_fp64": float(np.linalg.norm(v16 - ref) / np.linalg.norm(ref)),
+                   "max_abs_err_over_max_abs_vs_fp64": float(np.abs(v16 - ref).max() / np.abs(ref).max())},
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_hex125()
+    return out
+
+
+def hooke_symgrad(E, nu):
+    """6x6 Hooke matrix in SymGrad row order (test/test_dsl.jl:16-26)."""
+    mu = E / (2 * (1 + nu))
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] = lam + 2 * mu
+    D[np.arange(3, 6), np.arange(3, 6)] = mu
+    return D
+
+
+def cpu_baseline_hex125(n=6):
+    """Oracle direct-CSC FP64 stiffness of a 6^3 hex125 cube (same element,
+    rule and perturbation as the GPU workload), all threads."""
+    from lowlevelfem import mesh as M, refelem as R
+    from oracle import oracle as O
+    m = M.structured_box_mesh(n=n, order=4, perturb=0.05, seed=12)
+    tab = R.rule_tables(m.etype, 9)
+    cp, rv = O.csc_pattern(m)
+    T = O.num_threads()
+    t0 = time.perf_counter()
+    O.stiffness_csc(m, tab, hooke_symgrad(2e5, 0.3), cp, rv, nthreads=T)
+    t = time.perf_counter() - t0
+    return {"value": m.E / t, "unit": "elem/s", "cores": T, "kind": "port",
+            "sample": f"hex125 {n}^3 cube ({m.E} elements), oracle direct-CSC FP64 assembly, single pass, "
+                      f"{T} OpenMP threads on {cpu_model()}"}
+
+
+def bench_batch(args):
+    """256 seeded uneven problems, stiffness + consistent mass, one persistent
+    launch per step."""
+    from lowlevelfem import _lib, mesh as M, workloads as W
+    probs = W.batched_problems(seed=2024, count=256)
+    t0 = time.perf_counter()
+    meshes = [p.mesh() for p in probs]
+    log(f"batch: {len(probs)} problems, {sum(m.E for m in meshes)} elements, "
+        f"{sum(3 * m.N for m in meshes)} DoF (mesh generation {time.perf_counter() - t0:.1f} s)")
+    ctx = _lib.default_context()
+    rules = {k: W.rules(k) for k in W.KINDS}
+    tuned = {}
+    for kind, (elem, order) in W.KINDS.items():     # one tuning sample per element type
+        sm = M.structured_box_mesh(n=24 if order == 1 else 12, elem=elem, order=order)
+        r = _lib.DeviceMesh(ctx, sm.coords, sm.etype, sm.conn).autotune(_lib.OP_STIFFNESS, rules[kind][0])
+        tuned[kind] = {"mult": r.mult, "waves_per_block": r.waves_per_block,
+                       "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates}
+        log(f"autotune {kind}: {tuned[kind]}")
+    t0 = time.perf_counter()
+    devs = [_lib.DeviceMesh(ctx, m.coords, m.etype, m.conn) for m in meshes]
+    ctx.synchronize()
+    setup_s = time.perf_counter() - t0
+    items = []
+    for p, d in zip(probs, devs):
+        tk, tm = rules[p.kind]
+        items.append((d, _lib.OP_STIFFNESS, tk, dict(E=p.E, nu=p.nu)))
+        items.append((d, _lib.OP_MASS, tm, dict(rho=p.rho)))
+    nnzK = [d.pattern(_lib.OP_STIFFNESS, with_arrays=False) for d in devs]
+    for _ in range(args.warmup):
+        _lib.assemble_batch(items)
+    ks, ps = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = _lib.assemble_batch(items)
+        ks.append(st.kernel_ms)
+        ps.append(st.main_kernel_ms)
+    ctx.synchronize()
+    ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    E = sum(m.E for m in meshes)
+    alg = sum(24.0 * m.N + 4.0 * m.npe * m.E + 8.0 * nk + 8.0 * nk / 3 for m, nk in zip(meshes, nnzK))
+    k_ms = float(np.mean(ks))
+    achieved = alg / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched assembled elem/s (256 problems, FP64 stiffness + consistent mass, one launch)",
+        "value": E / (ms_step * 1e-3),
+        "unit": "elem/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 problems (seed 2024): hex8/tet10/wedge6 boxes, edges U[4,48], "
+                               "2k-400k DoF, E~U[1e5,3e5], nu~U[0.2,0.45], K + consistent M",
+                   "problems": len(probs), "elements": E, "dofs": sum(3 * m.N for m in meshes),
+                   "nnz_K": int(sum(nnzK))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernel_ms": {"batch_events": k_ms, "persistent_panel_kernel_events": float(np.mean(ps))},
+        "autotune": tuned,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(probs, meshes, rules)
+    return out
+
+
+def cpu_baseline_batch(probs, meshes, rules, budget_s=15.0):
+    """Oracle K (direct CSC, all threads) + consistent M (legacy IJV path) over
+    the first problems of the batch until about budget_s of CPU work."""
+    from oracle import oracle as O
+    T = O.num_threads()
+    done_e, t_tot, n = 0, 0.0, 0
+    for p, m in zip(probs, meshes):
+        tk, tm = rules[p.kind]
+        D = hooke_symgrad(p.E, p.nu)
+        cp, rv = O.csc_pattern(m)
+        t0 = time.perf_counter()
+        O.stiffness_csc(m, tk, D, cp, rv, nthreads=T)
+        O.mass_legacy(m, tm, rho=p.rho)
+        t_tot += time.perf_counter() - t0
+        done_e += m.E
+        n += 1
+        if t_tot > budget_s:
+            break
+    return {"value": done_e / t_tot, "unit": "elem/s", "cores": T, "kind": "port",
+            "sample": f"first {n} problems of the batch ({done_e} elements): oracle direct-CSC K with "
+                      f"{T} OpenMP threads + legacy-path consistent M, single pass, on {cpu_model()}"}
+
+
+def bench_tet10_solve(args):
+    """Assemble + traction + Dirichlet-eliminated PCG to relative residual
+    1e-10 on the tet10 beam: left face (x = 0) clamped, fy = -1 on the right
+    face (x = 1).  Plain FP64 Jacobi-PCG and FP32-inner / FP64-refined PCG."""
+    from lowlevelfem import _lib, mesh as M, refelem as R
+    cells = tuple(int(x) for x in args.cells.split(",")) if args.cells else CELLS
+    m = M.structured_box_mesh(lz=LZ * cells[2] / CELLS[2], cells=cells, elem="tet", order=2)
+    tab = R.rule_tables(11, 2)
+    ctx = _lib.default_context()
+    dev = _lib.DeviceMesh(ctx, m.coords, m.etype, m.conn)
+    nnz = dev.pattern(with_arrays=False)
+    _, sta = dev.assemble(_lib.OP_STIFFNESS, tab, copy_out=False)
+    face = m.faces["right"]
+    ftab = R.rule_tables(face.etype, 5)
+    f = dev.traction(face.etype, face.conn, ftab, [0.0, -1.0, 0.0])
+    left = m.nodes_on("left")
+    fixed = (3 * left[:, None] + np.arange(3)[None, :]).ravel()
+    res = {}
+    for name, method in (("fp64", _lib.PCG_JACOBI_FP64), ("mixed_restart", _lib.PCG_MIXED_REFINE),
+                         ("mixed_f32vec", _lib.PCG_MIXED_RELIABLE), ("mixed", _lib.PCG_F32OP_RELIABLE)):
+        t0 = time.perf_counter()
+        u, st = dev.solve(f, fixed, method=method, reltol=1e-10, maxiter=200000)
+        t = time.perf_counter() - t0
+        res[name] = {"seconds": t, "iterations": int(st.iterations), "refinements": int(st.refinements),
+                     "relres": float(st.relres), "sum_u": float(u.sum())}
+        log(f"solve {name}: {res[name]}")
+    nblk = nnz // 9
+    n = 3 * m.N
+    it64 = res["fp64"]["iterations"]
+    bytes64 = it64 * (nblk * (72 + 4) + 7 * 8 * n)
+    achieved = bytes64 / res["fp64"]["seconds"] / 1e9
+    return {
+        "metric": "seconds to relative residual 1e-10 (assembled K, traction load, Dirichlet-eliminated PCG)",
+        "value": res["mixed"]["seconds"], "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * res["mixed"]["seconds"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"tet10 Kuhn beam {cells[0]}x{cells[1]}x{cells[2]}, left clamped, fy=-1 on right, "
+                               "PCG to rel-res 1e-10 (value: FP32 operator, FP64 Krylov vectors, block-Jacobi, "
+                               "FP64 reliable updates)",
+                   "dofs": n, "nnz": nnz},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "plain FP64 PCG: per iteration 76 B per 3x3 block + 7 vectors of 8 B per DoF"},
+        "solves": res,
+        "assembly_kernel_ms": sta.kernel_ms,
+    }
+
+
+WORKLOADS = {"tet10-beam": bench_tet10, "hex125": bench_hex125, "batch256": bench_batch,
+             "tet10-solve": bench_tet10_solve}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="tet10-beam")
+    ap.add_argument("--cells", type=str, default=None, help="override the mesh size (testing only)")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="after the steps, copy this many bytes device-to-device once "
+                         "(reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+    out = WORKLOADS[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

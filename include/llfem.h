@@ -57,7 +57,11 @@ enum {
 
 enum {
   LLFEM_PCG_JACOBI_FP64 = 0,    /* Jacobi-PCG in FP64 */
-  LLFEM_PCG_MIXED_REFINE = 1    /* FP32 Jacobi-PCG inner solves + FP64 residual refinement */
+  LLFEM_PCG_MIXED_REFINE = 1,   /* FP32 Jacobi-PCG inner solves + FP64 residual refinement */
+  LLFEM_PCG_MIXED_RELIABLE = 2, /* FP32 block-Jacobi PCG with FP64 reliable updates (the true
+                                   FP64 residual replaces the recursive one every 10x reduction;
+                                   converged on ||b - K x|| <= reltol ||b||); llfem_solve only */
+  LLFEM_PCG_F32OP_RELIABLE = 3  /* as 2 with FP64 Krylov vectors: only the operator is FP32 */
 };
 
 /* One operator request.  The tables are what gmsh hands the reference:

@@ -179,7 +179,8 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
   const int npe = m->npe, q = d->q;
   const size_t geo_doubles = (size_t)kGeoStride * m->E * (affine ? 1 : q);
   LLFEM_TRY(ensure_buf(ctx, &m->d_geo, &m->geo_bytes, sizeof(double) * geo_doubles));
-  k_geometry<<<(unsigned)((m->E + 255) / 256), 256, 0, ctx->stream>>>(
+  const int64_t threads = affine ? m->E : m->E * q;
+  k_geometry<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,
       m->d_tab + (size_t)q * npe * 3, m->d_geo);
   return LLFEM_OK;
@@ -189,8 +190,14 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
 
 using namespace llfem;
 
-extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out,
-                              llfem_stats* stats) {
+namespace llfem {
+
+// llfem_assemble with an optional device target: dev_out != NULL receives the
+// values (pdim*nadj*blk doubles) instead of the mesh's own buffer, and the
+// mesh's assembled-operator state is left unchanged (used by the server,
+// whose concurrent requests on one mesh have their own materials).
+int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfem_stats* stats,
+                  double* dev_out) {
   auto t0 = std::chrono::steady_clock::now();
   if (!m || !d) return fail(LLFEM_EINVAL, "llfem_assemble: NULL argument");
   if (d->op != LLFEM_OP_STIFFNESS && d->op != LLFEM_OP_MASS)
@@ -239,7 +246,10 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
   double** outp = (op == LLFEM_OP_STIFFNESS) ? &m->d_valK : &m->d_valM;
   const int blk = (op == LLFEM_OP_STIFFNESS) ? 9 : 3;
   const int64_t nnz = m->nadj * blk;
-  if (!*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)nnz));
+  if (dev_out && op == LLFEM_OP_MASS && d->lumped)
+    return fail(LLFEM_EINVAL, "assemble: lumped mass has no device target");
+  if (!dev_out && !*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)nnz));
+  double* target = dev_out ? dev_out : *outp;
 
   PanelParams P{};
   P.rec_bytes = m->rec_bytes;
@@ -261,12 +271,12 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
           for (int be = 0; be < 3; ++be)
             P.C[((dd * 3 + al) * 3 + c) * 3 + be] = d->D[voigt(dd, al) * 6 + voigt(c, be)];
   P.rho = d->rho;
-  P.out = *outp;
+  P.out = target;
   const int npw = 64 / npe;
   const int blkl = (op == LLFEM_OP_STIFFNESS) ? 9 : 1;
   if (highorder)
     LLFEM_TRY(assemble_highorder(m, d, affine, m->d_tab, m->d_tab + (size_t)q * npe * 3, P.lam, P.mu,
-                                 iso ? nullptr : P.C, *outp));
+                                 iso ? nullptr : P.C, target));
   else
     LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
   for (const PanelLaunch& L : (highorder ? std::vector<PanelLaunch>() : m->launches)) {
@@ -309,7 +319,9 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
     k_lump<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_valM, m->d_lumped);
   }
   LLFEM_HIP(hipEventRecord(ctx->ev[3], st));
-  if (op == LLFEM_OP_STIFFNESS) {
+  if (dev_out) {
+    // the mesh's own operator buffers are untouched
+  } else if (op == LLFEM_OP_STIFFNESS) {
     m->have_K = true;
     m->solver_ready = false;
   } else {
@@ -321,7 +333,7 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
       LLFEM_HIP(hipMemcpyAsync(nzval_out, m->d_lumped, sizeof(double) * 3 * m->N,
                                hipMemcpyDeviceToHost, st));
     else
-      LLFEM_HIP(hipMemcpyAsync(nzval_out, *outp, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
+      LLFEM_HIP(hipMemcpyAsync(nzval_out, target, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
   }
   LLFEM_HIP(hipEventSynchronize(ctx->ev[3]));
   if (nzval_out) LLFEM_HIP(hipStreamSynchronize(st));
@@ -347,4 +359,11 @@ extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzv
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return LLFEM_OK;
+}
+
+}  // namespace llfem
+
+extern "C" int llfem_assemble(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out,
+                              llfem_stats* stats) {
+  return assemble_impl(m, d, nzval_out, stats, nullptr);
 }

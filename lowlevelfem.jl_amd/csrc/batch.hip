@@ -26,9 +26,9 @@ namespace {
 
 struct BatchSeg {
   PanelParams P;      // the panel launch this segment runs
-  int64_t wave0;      // first global work item of the segment
+  int64_t item0;      // first global work item of the segment
+  int32_t span;       // launch waves per work item (the last item may have fewer)
   int32_t kind;       // kernel variant (batch_kind)
-  int32_t pad;
 };
 
 struct GeoSeg {
@@ -55,12 +55,13 @@ int npe_index(int npe) {
 constexpr int batch_kind(int npe_idx, int op, bool affine) { return (npe_idx * 2 + op) * 2 + (affine ? 1 : 0); }
 
 template <int NPE, int OP, bool AFFINE>
-__device__ __forceinline__ void run_item(const PanelParams& P, int64_t w, double* wl) {
-  panel_wave<NPE, uint8_t, OP, AFFINE, true, double>(P, w, wl);
+__device__ __forceinline__ void run_item(const PanelParams& P, int64_t w0, int nw, double* wl) {
+  panel_wave<NPE, uint8_t, OP, AFFINE, true, double>(P, w0, nw, wl);
 }
 
 // Persistent panel kernel.  Each wavefront repeatedly takes the next work item
-// (one wavefront's share of one segment) from the queue; the item index is
+// (`span` consecutive waves of one segment, run as one stream) from the
+// queue; the item index is
 // made wave-uniform with readfirstlane, so the segment search and the
 // parameter loads are scalar.  Every wavefront exits once the queue passes
 // `total`.
@@ -78,27 +79,29 @@ __global__ __launch_bounds__(256) void k_panel_batch(const BatchSeg* __restrict_
     int lo = 0, hi = nseg - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (segs[mid].wave0 <= (int64_t)item) lo = mid; else hi = mid - 1;
+      if (segs[mid].item0 <= (int64_t)item) lo = mid; else hi = mid - 1;
     }
     const PanelParams P = segs[lo].P;
-    const int64_t w = (int64_t)item - segs[lo].wave0;
+    const int span = segs[lo].span;
+    const int64_t w = ((int64_t)item - segs[lo].item0) * span;
+    const int nw = (int)min((int64_t)span, P.waves - w);
     switch (segs[lo].kind) {
-      case batch_kind(0, 0, false): run_item<4, 0, false>(P, w, wl); break;
-      case batch_kind(0, 0, true): run_item<4, 0, true>(P, w, wl); break;
-      case batch_kind(0, 1, false): run_item<4, 1, false>(P, w, wl); break;
-      case batch_kind(0, 1, true): run_item<4, 1, true>(P, w, wl); break;
-      case batch_kind(1, 0, false): run_item<6, 0, false>(P, w, wl); break;
-      case batch_kind(1, 0, true): run_item<6, 0, true>(P, w, wl); break;
-      case batch_kind(1, 1, false): run_item<6, 1, false>(P, w, wl); break;
-      case batch_kind(1, 1, true): run_item<6, 1, true>(P, w, wl); break;
-      case batch_kind(2, 0, false): run_item<8, 0, false>(P, w, wl); break;
-      case batch_kind(2, 0, true): run_item<8, 0, true>(P, w, wl); break;
-      case batch_kind(2, 1, false): run_item<8, 1, false>(P, w, wl); break;
-      case batch_kind(2, 1, true): run_item<8, 1, true>(P, w, wl); break;
-      case batch_kind(3, 0, false): run_item<10, 0, false>(P, w, wl); break;
-      case batch_kind(3, 0, true): run_item<10, 0, true>(P, w, wl); break;
-      case batch_kind(3, 1, false): run_item<10, 1, false>(P, w, wl); break;
-      case batch_kind(3, 1, true): run_item<10, 1, true>(P, w, wl); break;
+      case batch_kind(0, 0, false): run_item<4, 0, false>(P, w, nw, wl); break;
+      case batch_kind(0, 0, true): run_item<4, 0, true>(P, w, nw, wl); break;
+      case batch_kind(0, 1, false): run_item<4, 1, false>(P, w, nw, wl); break;
+      case batch_kind(0, 1, true): run_item<4, 1, true>(P, w, nw, wl); break;
+      case batch_kind(1, 0, false): run_item<6, 0, false>(P, w, nw, wl); break;
+      case batch_kind(1, 0, true): run_item<6, 0, true>(P, w, nw, wl); break;
+      case batch_kind(1, 1, false): run_item<6, 1, false>(P, w, nw, wl); break;
+      case batch_kind(1, 1, true): run_item<6, 1, true>(P, w, nw, wl); break;
+      case batch_kind(2, 0, false): run_item<8, 0, false>(P, w, nw, wl); break;
+      case batch_kind(2, 0, true): run_item<8, 0, true>(P, w, nw, wl); break;
+      case batch_kind(2, 1, false): run_item<8, 1, false>(P, w, nw, wl); break;
+      case batch_kind(2, 1, true): run_item<8, 1, true>(P, w, nw, wl); break;
+      case batch_kind(3, 0, false): run_item<10, 0, false>(P, w, nw, wl); break;
+      case batch_kind(3, 0, true): run_item<10, 0, true>(P, w, nw, wl); break;
+      case batch_kind(3, 1, false): run_item<10, 1, false>(P, w, nw, wl); break;
+      case batch_kind(3, 1, true): run_item<10, 1, true>(P, w, nw, wl); break;
       default: break;
     }
   }
@@ -337,14 +340,30 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
       P.out = *outp;
       P.lds_per_wave = npw * L.max_r * blkl + 9;
       max_lds = std::max<size_t>(max_lds, (size_t)P.lds_per_wave);
-      s.wave0 = wtot;
       s.kind = batch_kind(npe_index(m->npe), op, it.affine);
       bsegs.push_back(s);
       wtot += L.waves;
     }
     alg_bytes += 24.0 * m->N + 4.0 * m->npe * m->E + 8.0 * m->nadj * blk;
   }
-  if (wtot >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "llfem_assemble_batch: too many work items");
+  // waves per block: 4 unless the widest panel does not fit
+  const size_t wave_bytes = sizeof(double) * max_lds;
+  if (!bsegs.empty() && wave_bytes > 160 * 1024)
+    return fail(LLFEM_EUNSUP, "llfem_assemble_batch: node panel exceeds LDS");
+  int wpb = 4;
+  while (wpb > 1 && wpb * wave_bytes > 160 * 1024) --wpb;
+  const size_t lds_bytes = wave_bytes * wpb;
+  const int per_cu = std::max<int>(1, (int)((160 * 1024) / std::max<size_t>(lds_bytes, 1)));
+  // work items: about 8 per resident wavefront, each `span` launch waves
+  const int64_t resident = (int64_t)std::max(1, ctx->num_cus) * per_cu * wpb;
+  const int64_t span = std::max<int64_t>(1, wtot / (8 * resident));
+  int64_t items_total = 0;
+  for (BatchSeg& s : bsegs) {
+    s.span = (int32_t)std::min<int64_t>(span, std::max<int64_t>(1, s.P.waves));
+    s.item0 = items_total;
+    items_total += (s.P.waves + s.span - 1) / s.span;
+  }
+  if (items_total >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "llfem_assemble_batch: too many work items");
 
   float ms_all = 0.f, ms_main = 0.f;
   if (!bsegs.empty()) {
@@ -357,14 +376,7 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     uint32_t* d_queue = reinterpret_cast<uint32_t*>(sc + qoff);
     LLFEM_HIP(hipMemcpyAsync(d_gsegs, gsegs.data(), gbytes, hipMemcpyHostToDevice, st));
     LLFEM_HIP(hipMemcpyAsync(d_bsegs, bsegs.data(), bbytes, hipMemcpyHostToDevice, st));
-    // waves per block: 4 unless the widest panel does not fit
-    const size_t wave_bytes = sizeof(double) * max_lds;
-    int wpb = 4;
-    while (wpb > 1 && wpb * wave_bytes > 160 * 1024) --wpb;
-    if (wave_bytes > 160 * 1024) return fail(LLFEM_EUNSUP, "llfem_assemble_batch: node panel exceeds LDS");
-    const size_t lds_bytes = wave_bytes * wpb;
-    const int per_cu = std::max<int>(1, (int)((160 * 1024) / lds_bytes));
-    const int64_t want = (wtot + wpb - 1) / wpb;
+    const int64_t want = (items_total + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per_cu, want));
     if (lds_bytes > 65536)
       LLFEM_HIP(hipFuncSetAttribute((const void*)k_panel_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -374,7 +386,7 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     k_zero_u32<<<1, 64, 0, st>>>(d_queue);
     LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
     hipLaunchKernelGGL(k_panel_batch, dim3(blocks), dim3(64 * wpb), lds_bytes, st, d_bsegs,
-                       (int)bsegs.size(), (uint32_t)wtot, d_queue, (int)max_lds);
+                       (int)bsegs.size(), (uint32_t)items_total, d_queue, (int)max_lds);
     LLFEM_HIP(hipGetLastError());
     LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
     for (Item& it : items) {

@@ -171,6 +171,8 @@ inline int etype_dim(int et) {
   }
 }
 
+int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfem_stats* stats,
+                  double* dev_out);
 std::vector<double> operator_table(const llfem_op_desc* d, int npe, bool affine);
 int voigt(int i, int j);
 
@@ -178,5 +180,60 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
                        const double* d_w, double lam, double mu, const double* C81, double* out);
 
 uint64_t hash_bytes(const void* p, size_t n, uint64_t h = 1469598103934665603ull);
+
+// ---- batched mixed-precision PCG engine (engine.hip) --------------------------
+// One linear system in node-panel form; all pointers are device memory owned
+// by the caller and must stay valid while the system is in the engine.
+struct SysSpec {
+  int64_t N = 0;                    // nodes (3 DoFs each)
+  int64_t nadj = 0;                 // node blocks of K
+  const int64_t* adj_ptr = nullptr;
+  const int32_t* adj = nullptr;
+  const int32_t* dslot = nullptr;   // diagonal slot of each node
+  const double* val64 = nullptr;    // FP64 panels of K
+  const uint8_t* fixed = nullptr;   // [3N] constrained-DoF mask
+  const double* f = nullptr;        // [3N] load
+  const double* ufix = nullptr;     // [3N] prescribed values (NULL = 0)
+  double reltol = 1e-10;
+  int64_t maxiter = 100000;
+};
+
+struct SolveSys;
+
+class BatchSolver {
+ public:
+  static constexpr double kDelta = 0.1;   // reliable-update trigger (residual reduction)
+  // f64_vectors: FP64 Krylov vectors over the FP32 operator (else FP32 vectors)
+  BatchSolver(llfem_ctx* c, bool f64_vectors);
+  ~BatchSolver();
+  static size_t bytes_needed(int64_t N, int64_t nadj, bool f64_vectors);
+  int add(const SysSpec& spec, SolveSys** out);               // set up and start a system
+  int run(int iters, std::vector<SolveSys*>* finished);       // iterations on all running systems
+  int finish(SolveSys* s, double* u);                          // write u (device), true residual
+  void remove(SolveSys* s);
+  bool any_running() const;
+
+ private:
+  int update(SolveSys* s, bool init);
+  int rebuild_tables();
+  void release(SolveSys* s);
+  llfem_ctx* ctx_;
+  bool f64v_;
+  std::vector<SolveSys*> sys_, live_;
+  bool dirty_ = true;
+  void* tables_ = nullptr;
+  size_t tables_bytes_ = 0;
+  void* d_sys_ = nullptr;
+  int32_t* d_sb_ = nullptr;
+  int32_t* d_vb_ = nullptr;
+  double* d_states_ = nullptr;
+  int nsys_ = 0;
+  int64_t nsb_ = 0, nvb_ = 0;
+};
+
+int64_t sys_iterations(const SolveSys* s);
+int64_t sys_updates(const SolveSys* s);
+int sys_status(const SolveSys* s);      // 0 running, 1 converged, -1 not converged (maxiter)
+double sys_relres(const SolveSys* s);
 
 }  // namespace llfem

@@ -1,0 +1,693 @@
+// Batched mixed-precision PCG engine: FP32 Krylov iterations with FP64
+// reliable updates, 3x3 block-Jacobi preconditioner, many systems per launch.
+//
+// Reference: solveDisplacement(K, f; support, iterative=true, reltol, ...)
+// (src/linear.jl:3711-3743): u = u_fixed on the constrained DoFs and
+// K_ff u_f = f_f - K_fc u_c by preconditioned CG, converged at
+// ||r|| <= reltol ||r0||.  Here the convergence test is made on the TRUE
+// FP64 residual ||b - K x|| <= reltol ||b|| (b = the eliminated right-hand
+// side), so the mixed-precision result meets the same relative residual as
+// the FP64 solver.
+//
+// Algorithm (reliable updates, Sleijpen & van der Vorst 1996, as used for
+// mixed-precision CG in lattice QCD solvers):
+//   * CG runs in FP32 on K32 (the assembled panels rounded to FP32) with the
+//     residual scaled to unit norm; dot products accumulate in FP64.
+//   * The FP32 correction x32 is folded into the FP64 solution and the true
+//     residual r = b - K x is recomputed in FP64 whenever the recursive
+//     residual has dropped by delta (0.1) since the last update, or reaches
+//     the tolerance.  The search direction p is kept (rescaled), so the
+//     Krylov process is not restarted and the iteration count stays close to
+//     the FP64 solver's.
+//
+// Batching: every launch covers all RUNNING systems.  Blocks map to systems
+// through per-launch tables; each system's rows are split into blocks that
+// depend on that system only and its dot products reduce in a fixed order,
+// so a system's iterates are bit-identical whether it runs alone or among
+// others.  A system that needs a reliable update pauses (its blocks exit at
+// once) until the host performs the update between chunks of iterations.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace llfem {
+
+namespace {
+
+constexpr int kSB = 256;                 // threads per block
+constexpr int kSG = 8;                   // lanes per node row in the SpMV
+constexpr int kSpmvNodes = kSB / kSG;    // nodes per SpMV block
+constexpr int kVecNodes = kSB;           // nodes per vector block
+
+enum { SC_RHO = 0, SC_ALPHA, SC_BETA, SC_RR, SC_MAXRR, SC_TOL2, SC_DELTA2, SC_STATE, SC_ITER, SC_PQ, SC_N = 16 };
+enum { ST_RUN = 0, ST_PENDING = 1, ST_PAUSED = 2, ST_IDLE = 3 };
+
+struct SysDev {
+  int64_t N;               // nodes (3 DoFs each)
+  const int64_t* adj_ptr;
+  const int32_t* adj;
+  const float* val;        // FP32 node blocks, block-contiguous: [adj slot][c][d]
+  const uint8_t* fixed;    // [3N] 1 = constrained
+  const float* binv;       // [N][9] inverse diagonal blocks (free part)
+  void *x, *r, *z, *p, *q; // Krylov vectors, FP32 or FP64 (the kernels' VT)
+  double* scal;            // [SC_N]
+  double* part;            // [nsblk] SpMV partials, then [2 * nvblk]
+  int32_t sblk0, nsblk, vblk0, nvblk;
+};
+
+__device__ __forceinline__ double bsum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// q = mask(K32 p) for RUNNING systems; partial p.q per block.
+template <typename VT>
+__global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys, const int32_t* __restrict__ bsys) {
+  __shared__ double sh[kSB];
+  const SysDev& S = sys[bsys[blockIdx.x]];
+  if (S.scal[SC_STATE] != ST_RUN) return;   // block-uniform
+  const VT* __restrict__ pv = static_cast<const VT*>(S.p);
+  VT* __restrict__ qv = static_cast<VT*>(S.q);
+  const int lb = blockIdx.x - S.sblk0;
+  const int64_t j = (int64_t)lb * kSpmvNodes + threadIdx.x / kSG;
+  const int l = threadIdx.x % kSG;
+  VT a0 = 0, a1 = 0, a2 = 0;
+  if (j < S.N) {
+    const int64_t p0 = S.adj_ptr[j];
+    const int R = (int)(S.adj_ptr[j + 1] - p0);
+    const float* P = S.val + 9 * p0;
+    for (int t = l; t < R; t += kSG) {
+      const int64_t i = S.adj[p0 + t];
+      const VT x0 = pv[3 * i], x1 = pv[3 * i + 1], x2 = pv[3 * i + 2];
+      const float* B = P + 9 * t;   // block t: [c][d], 36 contiguous bytes
+      a0 += (VT)B[0] * x0 + (VT)B[1] * x1 + (VT)B[2] * x2;
+      a1 += (VT)B[3] * x0 + (VT)B[4] * x1 + (VT)B[5] * x2;
+      a2 += (VT)B[6] * x0 + (VT)B[7] * x1 + (VT)B[8] * x2;
+    }
+  }
+#pragma unroll
+  for (int o = kSG / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, kSG);
+    a1 += __shfl_xor(a1, o, kSG);
+    a2 += __shfl_xor(a2, o, kSG);
+  }
+  double d = 0.0;
+  if (j < S.N && l == 0) {
+    const VT a[3] = {a0, a1, a2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const VT v = S.fixed[3 * j + c] ? (VT)0 : a[c];
+      qv[3 * j + c] = v;
+      d += (double)pv[3 * j + c] * (double)v;
+    }
+  }
+  d = bsum(d, sh);
+  if (threadIdx.x == 0) S.part[lb] = d;
+}
+
+// alpha = rho / p.q (one block per system).  A system flagged PENDING at the
+// end of the previous iteration becomes PAUSED here.
+__global__ __launch_bounds__(kSB) void k_bs_alpha(const SysDev* __restrict__ sys) {
+  __shared__ double sh[kSB];
+  const SysDev& S = sys[blockIdx.x];
+  const double st = S.scal[SC_STATE];
+  if (st == ST_PENDING) {
+    if (threadIdx.x == 0) S.scal[SC_STATE] = ST_PAUSED;
+    return;
+  }
+  if (st != ST_RUN) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < S.nsblk; i += kSB) v += S.part[i];
+  v = bsum(v, sh);
+  if (threadIdx.x == 0) {
+    S.scal[SC_PQ] = v;
+    if (v > 0.0) {
+      S.scal[SC_ALPHA] = S.scal[SC_RHO] / v;
+    } else {   // breakdown (p = 0 or K not SPD on the free set): check the true residual
+      S.scal[SC_ALPHA] = 0.0;
+      S.scal[SC_STATE] = ST_PAUSED;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha q; z = Binv r; partials (r.r, r.z).  Thread per node.
+template <typename VT>
+__global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sys, const int32_t* __restrict__ bsys) {
+  __shared__ double sh[kSB];
+  const SysDev& S = sys[bsys[blockIdx.x]];
+  if (S.scal[SC_STATE] != ST_RUN) return;
+  VT* __restrict__ xv = static_cast<VT*>(S.x);
+  VT* __restrict__ rv = static_cast<VT*>(S.r);
+  VT* __restrict__ zv = static_cast<VT*>(S.z);
+  const VT* __restrict__ pv = static_cast<const VT*>(S.p);
+  const VT* __restrict__ qv = static_cast<const VT*>(S.q);
+  const int lb = blockIdx.x - S.vblk0;
+  const int64_t j = (int64_t)lb * kVecNodes + threadIdx.x;
+  double rr = 0.0, rz = 0.0;
+  if (j < S.N) {
+    const VT alpha = (VT)S.scal[SC_ALPHA];
+    VT r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * j + c;
+      xv[i] += alpha * pv[i];
+      r[c] = rv[i] - alpha * qv[i];
+      rv[i] = r[c];
+    }
+    const float* B = S.binv + 9 * j;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const VT z = (VT)B[3 * c] * r[0] + (VT)B[3 * c + 1] * r[1] + (VT)B[3 * c + 2] * r[2];
+      zv[3 * j + c] = z;
+      rr += (double)r[c] * (double)r[c];
+      rz += (double)r[c] * (double)z;
+    }
+  }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    S.part[S.nsblk + 2 * lb] = rr;
+    S.part[S.nsblk + 2 * lb + 1] = rz;
+  }
+}
+
+// beta = rz / rho; reliable-update / convergence triggers (one block per system).
+__global__ __launch_bounds__(kSB) void k_bs_beta(const SysDev* __restrict__ sys) {
+  __shared__ double sh[kSB];
+  const SysDev& S = sys[blockIdx.x];
+  if (S.scal[SC_STATE] != ST_RUN) return;
+  double rr = 0.0, rz = 0.0;
+  for (int i = threadIdx.x; i < S.nvblk; i += kSB) {
+    rr += S.part[S.nsblk + 2 * i];
+    rz += S.part[S.nsblk + 2 * i + 1];
+  }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    const double rho = S.scal[SC_RHO];
+    S.scal[SC_BETA] = rho != 0.0 ? rz / rho : 0.0;
+    S.scal[SC_RHO] = rz;
+    S.scal[SC_RR] = rr;
+    S.scal[SC_ITER] += 1.0;
+    if (rr <= S.scal[SC_TOL2] || rr < S.scal[SC_DELTA2] * S.scal[SC_MAXRR] || !(rz > 0.0))
+      S.scal[SC_STATE] = ST_PENDING;
+  }
+}
+
+// p = z + beta p (RUNNING, and PENDING systems finishing their iteration).
+template <typename VT>
+__global__ __launch_bounds__(kSB) void k_bs_pupd(const SysDev* __restrict__ sys, const int32_t* __restrict__ bsys) {
+  const SysDev& S = sys[bsys[blockIdx.x]];
+  const double st = S.scal[SC_STATE];
+  if (st != ST_RUN && st != ST_PENDING) return;
+  VT* __restrict__ pv = static_cast<VT*>(S.p);
+  const VT* __restrict__ zv = static_cast<const VT*>(S.z);
+  const int lb = blockIdx.x - S.vblk0;
+  const int64_t j = (int64_t)lb * kVecNodes + threadIdx.x;
+  if (j >= S.N) return;
+  const VT beta = (VT)S.scal[SC_BETA];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pv[3 * j + c] = zv[3 * j + c] + beta * pv[3 * j + c];
+}
+
+__global__ void k_bs_states(const SysDev* __restrict__ sys, int nsys, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  out[2 * s] = sys[s].scal[SC_STATE];
+  out[2 * s + 1] = sys[s].scal[SC_ITER];
+}
+
+// ---- single-system kernels of the reliable update ------------------------------
+
+template <typename VT>
+__global__ void k_ru_accum(int64_t n, VT* __restrict__ x, double inv_s, double* __restrict__ xh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xh[i] += (double)x[i] * inv_s;
+  x[i] = (VT)0;
+}
+
+// y = mask(K64 x) (mask may be NULL)
+__global__ __launch_bounds__(kSB) void k_spmv64(int64_t N, const int64_t* __restrict__ adj_ptr,
+                                                const int32_t* __restrict__ adj, const double* __restrict__ val,
+                                                const double* __restrict__ x, const uint8_t* __restrict__ fixed,
+                                                double* __restrict__ y) {
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = gt / kSG;
+  const int l = (int)(gt % kSG);
+  double a0 = 0, a1 = 0, a2 = 0;
+  if (j < N) {
+    const int64_t p0 = adj_ptr[j];
+    const int R = (int)(adj_ptr[j + 1] - p0);
+    const double* P = val + 9 * p0;
+    for (int t = l; t < R; t += kSG) {
+      const int64_t i = adj[p0 + t];
+      const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+      a0 += P[3 * t] * x0 + P[3 * t + 1] * x1 + P[3 * t + 2] * x2;
+      a1 += P[3 * R + 3 * t] * x0 + P[3 * R + 3 * t + 1] * x1 + P[3 * R + 3 * t + 2] * x2;
+      a2 += P[6 * R + 3 * t] * x0 + P[6 * R + 3 * t + 1] * x1 + P[6 * R + 3 * t + 2] * x2;
+    }
+  }
+#pragma unroll
+  for (int o = kSG / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, kSG);
+    a1 += __shfl_xor(a1, o, kSG);
+    a2 += __shfl_xor(a2, o, kSG);
+  }
+  if (j < N && l == 0) {
+    const double a[3] = {a0, a1, a2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[3 * j + c] = (fixed && fixed[3 * j + c]) ? 0.0 : a[c];
+  }
+}
+
+// out = mask(b - y) (y may be NULL: out = mask(b)); partial sum of out^2 per block
+__global__ __launch_bounds__(kSB) void k_ru_resid(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
+                                                  const uint8_t* __restrict__ fixed, double* __restrict__ out,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[kSB];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    v = fixed[i] ? 0.0 : b[i] - (y ? y[i] : 0.0);
+    if (out) out[i] = v;
+  }
+  const double s = bsum(v * v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kSB) void k_sum(int64_t n, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[kSB];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kSB) v += part[i];
+  v = bsum(v, sh);
+  if (threadIdx.x == 0) out[0] = v;
+}
+
+// r = rh * s; p = pscale * p (= z when init); z = Binv r; partials (r.r, r.z)
+template <typename VT>
+__global__ __launch_bounds__(kSB) void k_ru_restart(int64_t N, const double* __restrict__ rh, double s, double pscale,
+                                                    int init, const float* __restrict__ binv, VT* __restrict__ r32,
+                                                    VT* __restrict__ z32, VT* __restrict__ p32,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[kSB];
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rz = 0.0;
+  if (j < N) {
+    VT r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r[c] = (VT)(rh[3 * j + c] * s);
+      r32[3 * j + c] = r[c];
+    }
+    const float* B = binv + 9 * j;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const VT z = (VT)B[3 * c] * r[0] + (VT)B[3 * c + 1] * r[1] + (VT)B[3 * c + 2] * r[2];
+      z32[3 * j + c] = z;
+      p32[3 * j + c] = init ? z : (VT)pscale * p32[3 * j + c];
+      rr += (double)r[c] * (double)r[c];
+      rz += (double)r[c] * (double)z;
+    }
+  }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = rr;
+    part[2 * blockIdx.x + 1] = rz;
+  }
+}
+
+__global__ __launch_bounds__(kSB) void k_ru_scal(int64_t nb, const double* __restrict__ part, double* __restrict__ scal,
+                                                 double tol2, double delta2) {
+  __shared__ double sh[kSB];
+  double rr = 0.0, rz = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += kSB) {
+    rr += part[2 * i];
+    rz += part[2 * i + 1];
+  }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    scal[SC_RHO] = rz;
+    scal[SC_RR] = rr;
+    scal[SC_MAXRR] = rr;
+    scal[SC_TOL2] = tol2;
+    scal[SC_DELTA2] = delta2;
+    scal[SC_BETA] = 0.0;
+    scal[SC_STATE] = ST_RUN;
+  }
+}
+
+// Jacobi preconditioner, stored as 3x3 blocks: the inverse diagonal of K on
+// the free components, zero elsewhere.  (Scalar Jacobi converged faster than
+// the inverse 3x3 diagonal blocks on the elasticity systems measured; the
+// block storage lets the update kernel take either.)
+__global__ void k_binv(int64_t N, const int64_t* __restrict__ adj_ptr, const int32_t* __restrict__ dslot,
+                       const double* __restrict__ val, const uint8_t* __restrict__ fixed, float* __restrict__ binv) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const int64_t a0 = adj_ptr[j];
+  const int64_t R = adj_ptr[j + 1] - a0;
+  const int64_t s = dslot[j];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double kdd = val[9 * a0 + d * 3 * R + 3 * s + d];   // K[(j,d),(j,d)] = panel[d][s][d]
+    const bool free = fixed[3 * j + d] == 0 && kdd != 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) binv[9 * j + d * 3 + c] = (c == d && free) ? (float)(1.0 / kdd) : 0.f;
+  }
+}
+
+// FP32 operator in block-contiguous order: out[(a0+t)*9 + c*3 + d] = panel
+// entry [c][t][d] of node j (thread per node block).
+__global__ void k_to32_blocks(int64_t N, const int64_t* __restrict__ adj_ptr, const double* __restrict__ val,
+                              float* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+  if (j >= N) return;
+  const int64_t a0 = adj_ptr[j];
+  const int64_t R = adj_ptr[j + 1] - a0;
+  const double* P = val + 9 * a0;
+  for (int64_t t = threadIdx.x; t < R; t += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[9 * (a0 + t) + 3 * c + d] = (float)P[c * 3 * R + 3 * t + d];
+}
+
+__global__ void k_fill_u(int64_t n, const uint8_t* __restrict__ fixed, const double* __restrict__ ufix,
+                         const double* __restrict__ x, double* __restrict__ u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u[i] = fixed[i] ? (ufix ? ufix[i] : 0.0) : (x ? x[i] : 0.0);
+}
+
+inline unsigned blocks(int64_t n, int bs = kSB) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct SolveSys {
+  SysSpec spec;
+  int64_t n = 0, nadj = 0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  double *b = nullptr, *xh = nullptr, *rh = nullptr, *y = nullptr, *scal = nullptr, *part = nullptr;
+  void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;   // Krylov vectors (VT)
+  float *binv = nullptr, *val32 = nullptr;
+  int64_t npart = 0;
+  double bnorm = 0.0, scale = 1.0, relres = 0.0;
+  int64_t iters = 0, updates = 0;
+  int status = 0;           // 0 running, 1 converged, -1 failed
+  int32_t sblk0 = 0, nsblk = 0, vblk0 = 0, nvblk = 0;
+};
+
+BatchSolver::BatchSolver(llfem_ctx* c, bool f64_vectors) : ctx_(c), f64v_(f64_vectors) {}
+
+BatchSolver::~BatchSolver() {
+  for (SolveSys* s : sys_) release(s);
+  sys_.clear();
+  if (tables_) dfree(ctx_, tables_, tables_bytes_);
+}
+
+void BatchSolver::release(SolveSys* s) {
+  if (s->mem) dfree(ctx_, s->mem, s->bytes);
+  delete s;
+}
+
+size_t BatchSolver::bytes_needed(int64_t N, int64_t nadj, bool f64_vectors) {
+  const int64_t n = 3 * N;
+  const int64_t nsblk = (N + kSpmvNodes - 1) / kSpmvNodes, nvblk = (N + kVecNodes - 1) / kVecNodes;
+  const int64_t npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
+  return sizeof(double) * (4 * (size_t)n + SC_N + (size_t)npart) +
+         (f64_vectors ? sizeof(double) : sizeof(float)) * 5 * (size_t)n +
+         sizeof(float) * (9 * (size_t)N + 9 * (size_t)nadj) + 16 * 256;   // 13 buffers, 256-B aligned
+}
+
+// Reliable update of one system: fold x32 into xh, recompute the FP64
+// residual, test convergence, restart the FP32 recursion.
+int BatchSolver::update(SolveSys* s, bool init) {
+  hipStream_t st = ctx_->stream;
+  const SysSpec& sp = s->spec;
+  const int64_t n = s->n;
+  if (!init) {
+    if (f64v_) k_ru_accum<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh);
+    else k_ru_accum<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh);
+    k_spmv64<<<blocks(sp.N * kSG), kSB, 0, st>>>(sp.N, sp.adj_ptr, sp.adj, sp.val64, s->xh, sp.fixed, s->y);
+    k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, s->y, sp.fixed, s->rh, s->part);
+  } else {
+    k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, nullptr, sp.fixed, s->rh, s->part);
+  }
+  k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 1);
+  double r2 = 0.0, it = 0.0;
+  LLFEM_HIP(hipMemcpyAsync(&r2, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipMemcpyAsync(&it, s->scal + SC_ITER, sizeof(double), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  const double rn = std::sqrt(r2);
+  s->iters = (int64_t)it;
+  if (init) s->bnorm = rn;
+  else ++s->updates;
+  if (rn <= sp.reltol * s->bnorm || s->bnorm == 0.0) {
+    s->status = 1;
+    const double idle = ST_IDLE;
+    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  }
+  if (s->iters >= sp.maxiter || s->updates > 100000) {
+    s->status = -1;
+    s->relres = s->bnorm > 0 ? rn / s->bnorm : 0.0;
+    const double idle = ST_IDLE;
+    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  }
+  const double s_new = 1.0 / rn;
+  const double pscale = s_new / s->scale;
+  s->scale = s_new;
+  const double tol = sp.reltol * s->bnorm * s_new;
+  const int64_t nb = blocks(sp.N);
+  if (f64v_)
+    k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, init ? 1 : 0, s->binv,
+                                                       (double*)s->r, (double*)s->z, (double*)s->p, s->part);
+  else
+    k_ru_restart<float><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, init ? 1 : 0, s->binv,
+                                                      (float*)s->r, (float*)s->z, (float*)s->p, s->part);
+  k_ru_scal<<<1, kSB, 0, st>>>(nb, s->part, s->scal, tol * tol, kDelta * kDelta);
+  LLFEM_HIP(hipGetLastError());
+  return LLFEM_OK;
+}
+
+int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
+  hipStream_t st = ctx_->stream;
+  SolveSys* s = new SolveSys();
+  s->spec = spec;
+  s->n = 3 * spec.N;
+  s->nadj = spec.nadj;
+  s->bytes = bytes_needed(spec.N, spec.nadj, f64v_);
+  int rc = dmalloc(ctx_, &s->mem, s->bytes);
+  if (rc != LLFEM_OK) {
+    delete s;
+    return rc;
+  }
+  const int64_t n = s->n, N = spec.N;
+  const int64_t nsblk = (N + kSpmvNodes - 1) / kSpmvNodes, nvblk = (N + kVecNodes - 1) / kVecNodes;
+  s->npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
+  char* m = static_cast<char*>(s->mem);
+  auto take = [&](size_t bytes) -> char* {
+    char* p = m;
+    m += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  s->b = (double*)take(sizeof(double) * n);
+  s->xh = (double*)take(sizeof(double) * n);
+  s->rh = (double*)take(sizeof(double) * n);
+  s->y = (double*)take(sizeof(double) * n);
+  s->scal = (double*)take(sizeof(double) * SC_N);
+  s->part = (double*)take(sizeof(double) * s->npart);
+  const size_t vsz = f64v_ ? sizeof(double) : sizeof(float);
+  s->x = take(vsz * n);
+  s->r = take(vsz * n);
+  s->z = take(vsz * n);
+  s->p = take(vsz * n);
+  s->q = take(vsz * n);
+  s->binv = (float*)take(sizeof(float) * 9 * N);
+  s->val32 = (float*)take(sizeof(float) * 9 * spec.nadj);
+  if ((size_t)(m - static_cast<char*>(s->mem)) > s->bytes) {
+    release(s);
+    return fail(LLFEM_EINVAL, "engine: workspace layout overflow");
+  }
+  // b = mask(f - K u_fixed); x = 0; FP32 operator and block-Jacobi inverse
+  k_fill_u<<<blocks(n), kSB, 0, st>>>(n, spec.fixed, spec.ufix, nullptr, s->y);
+  k_spmv64<<<blocks(N * kSG), kSB, 0, st>>>(N, spec.adj_ptr, spec.adj, spec.val64, s->y, nullptr, s->rh);
+  k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, spec.f, s->rh, spec.fixed, s->b, s->part);
+  k_to32_blocks<<<blocks(N, 8), dim3(32, 8), 0, st>>>(N, spec.adj_ptr, spec.val64, s->val32);
+  k_binv<<<blocks(N), kSB, 0, st>>>(N, spec.adj_ptr, spec.dslot, spec.val64, spec.fixed, s->binv);
+  LLFEM_HIP(hipMemsetAsync(s->xh, 0, sizeof(double) * n, st));
+  LLFEM_HIP(hipMemsetAsync(s->x, 0, vsz * n, st));
+  LLFEM_HIP(hipMemsetAsync(s->scal, 0, sizeof(double) * SC_N, st));
+  s->scale = 1.0;
+  rc = update(s, true);
+  if (rc != LLFEM_OK) {
+    release(s);
+    return rc;
+  }
+  sys_.push_back(s);
+  dirty_ = true;
+  *out = s;
+  return LLFEM_OK;
+}
+
+int BatchSolver::rebuild_tables() {
+  std::vector<SysDev> dev;
+  std::vector<int32_t> sb, vb;
+  live_.clear();
+  for (SolveSys* s : sys_) {
+    if (s->status != 0) continue;
+    const SysSpec& sp = s->spec;
+    SysDev d{};
+    d.N = sp.N;
+    d.adj_ptr = sp.adj_ptr;
+    d.adj = sp.adj;
+    d.val = s->val32;
+    d.fixed = sp.fixed;
+    d.binv = s->binv;
+    d.x = s->x; d.r = s->r; d.z = s->z; d.p = s->p; d.q = s->q;
+    d.scal = s->scal;
+    d.part = s->part;
+    d.nsblk = (int32_t)((sp.N + kSpmvNodes - 1) / kSpmvNodes);
+    d.nvblk = (int32_t)((sp.N + kVecNodes - 1) / kVecNodes);
+    d.sblk0 = (int32_t)sb.size();
+    d.vblk0 = (int32_t)vb.size();
+    const int32_t id = (int32_t)dev.size();
+    sb.insert(sb.end(), (size_t)d.nsblk, id);
+    vb.insert(vb.end(), (size_t)d.nvblk, id);
+    dev.push_back(d);
+    live_.push_back(s);
+  }
+  nsys_ = (int)dev.size();
+  nsb_ = (int64_t)sb.size();
+  nvb_ = (int64_t)vb.size();
+  const size_t need = sizeof(SysDev) * dev.size() + sizeof(int32_t) * (sb.size() + vb.size()) +
+                      sizeof(double) * 2 * dev.size() + 1024;
+  if (tables_bytes_ < need) {
+    if (tables_) dfree(ctx_, tables_, tables_bytes_);
+    tables_ = nullptr;
+    tables_bytes_ = 0;
+    LLFEM_TRY(dmalloc(ctx_, &tables_, need * 2));
+    tables_bytes_ = need * 2;
+  }
+  char* m = static_cast<char*>(tables_);
+  d_sys_ = reinterpret_cast<SysDev*>(m);
+  m += (sizeof(SysDev) * dev.size() + 255) / 256 * 256;
+  d_sb_ = reinterpret_cast<int32_t*>(m);
+  m += (sizeof(int32_t) * sb.size() + 255) / 256 * 256;
+  d_vb_ = reinterpret_cast<int32_t*>(m);
+  m += (sizeof(int32_t) * vb.size() + 255) / 256 * 256;
+  d_states_ = reinterpret_cast<double*>(m);
+  hipStream_t st = ctx_->stream;
+  if (!dev.empty()) {
+    LLFEM_HIP(hipMemcpyAsync(d_sys_, dev.data(), sizeof(SysDev) * dev.size(), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipMemcpyAsync(d_sb_, sb.data(), sizeof(int32_t) * sb.size(), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipMemcpyAsync(d_vb_, vb.data(), sizeof(int32_t) * vb.size(), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipStreamSynchronize(st));   // host vectors go out of scope
+  }
+  dirty_ = false;
+  return LLFEM_OK;
+}
+
+int BatchSolver::run(int iters, std::vector<SolveSys*>* finished) {
+  hipStream_t st = ctx_->stream;
+  if (dirty_) LLFEM_TRY(rebuild_tables());
+  if (nsys_ == 0) return LLFEM_OK;
+  const SysDev* sys = static_cast<const SysDev*>(d_sys_);
+  for (int k = 0; k < iters; ++k) {
+    if (f64v_) {
+      k_bs_spmv<double><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
+      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_update<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_pupd<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+    } else {
+      k_bs_spmv<float><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
+      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_update<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_pupd<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+    }
+  }
+  k_bs_states<<<blocks(nsys_), kSB, 0, st>>>(sys, nsys_, d_states_);
+  std::vector<double> h(2 * (size_t)nsys_);
+  LLFEM_HIP(hipMemcpyAsync(h.data(), d_states_, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  LLFEM_HIP(hipGetLastError());
+  for (int i = 0; i < nsys_; ++i) {
+    SolveSys* s = live_[i];
+    s->iters = (int64_t)h[2 * i + 1];
+    const int state = (int)h[2 * i];
+    if (state == ST_PENDING || state == ST_PAUSED) {
+      LLFEM_TRY(update(s, false));
+      if (s->status != 0) {
+        dirty_ = true;
+        if (finished) finished->push_back(s);
+      }
+    }
+  }
+  return LLFEM_OK;
+}
+
+// u = u_fixed on constrained DoFs, xh elsewhere; true relative residual
+// ||(K u - f)_free|| / ||f_free|| (test/test_dsl.jl:106).
+int BatchSolver::finish(SolveSys* s, double* u) {
+  hipStream_t st = ctx_->stream;
+  const SysSpec& sp = s->spec;
+  const int64_t n = s->n;
+  k_fill_u<<<blocks(n), kSB, 0, st>>>(n, sp.fixed, sp.ufix, s->xh, u);
+  k_spmv64<<<blocks(sp.N * kSG), kSB, 0, st>>>(sp.N, sp.adj_ptr, sp.adj, sp.val64, u, nullptr, s->y);
+  k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, sp.f, s->y, sp.fixed, nullptr, s->part);
+  k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 1);
+  k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, sp.f, nullptr, sp.fixed, nullptr, s->part);
+  k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 2);
+  double h[2];
+  LLFEM_HIP(hipMemcpyAsync(h, s->scal + SC_N - 2, sizeof(h), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  LLFEM_HIP(hipGetLastError());
+  s->relres = h[0] > 0 ? std::sqrt(h[1] / h[0]) : std::sqrt(h[1]);
+  return LLFEM_OK;
+}
+
+void BatchSolver::remove(SolveSys* s) {
+  auto it = std::find(sys_.begin(), sys_.end(), s);
+  if (it == sys_.end()) return;
+  sys_.erase(it);
+  (void)hipStreamSynchronize(ctx_->stream);
+  release(s);
+  dirty_ = true;
+}
+
+bool BatchSolver::any_running() const {
+  for (SolveSys* s : sys_)
+    if (s->status == 0) return true;
+  return false;
+}
+
+int64_t sys_iterations(const SolveSys* s) { return s->iters; }
+int64_t sys_updates(const SolveSys* s) { return s->updates; }
+int sys_status(const SolveSys* s) { return s->status; }
+double sys_relres(const SolveSys* s) { return s->relres; }
+
+}  // namespace llfem

@@ -89,6 +89,19 @@ __global__ void k_geom_check(int64_t E, int64_t N, int npe, int q, const double*
 // (element, point), wd = |det J_p| w_p.
 constexpr int kGeoStride = 16;   // doubles
 
+__device__ __forceinline__ void geometry_store(double* g, const double* Mi, double wd) {
+  double2* g2 = reinterpret_cast<double2*>(g);
+  g2[0] = make_double2(Mi[0], Mi[1]);
+  g2[1] = make_double2(Mi[2], Mi[3]);
+  g2[2] = make_double2(Mi[4], Mi[5]);
+  g2[3] = make_double2(Mi[6], Mi[7]);
+  g2[4] = make_double2(Mi[8], wd);
+  float4* f4 = reinterpret_cast<float4*>(g + 10);
+  f4[0] = make_float4((float)Mi[0], (float)Mi[1], (float)Mi[2], (float)Mi[3]);
+  f4[1] = make_float4((float)Mi[4], (float)Mi[5], (float)Mi[6], (float)Mi[7]);
+  f4[2] = make_float4((float)Mi[8], (float)wd, 0.f, 0.f);
+}
+
 __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int q, int affine,
                                              const double* __restrict__ xyz,
                                              const int32_t* __restrict__ conn,
@@ -101,27 +114,29 @@ __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int 
     jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
     const double det = inv3(J, Mi);
     const double wd = affine ? fabs(det) : fabs(det) * w[p];
-    double* g = geo + ((size_t)e * np + p) * kGeoStride;
-    double2* g2 = reinterpret_cast<double2*>(g);
-    g2[0] = make_double2(Mi[0], Mi[1]);
-    g2[1] = make_double2(Mi[2], Mi[3]);
-    g2[2] = make_double2(Mi[4], Mi[5]);
-    g2[3] = make_double2(Mi[6], Mi[7]);
-    g2[4] = make_double2(Mi[8], wd);
-    float4* f4 = reinterpret_cast<float4*>(g + 10);
-    f4[0] = make_float4((float)Mi[0], (float)Mi[1], (float)Mi[2], (float)Mi[3]);
-    f4[1] = make_float4((float)Mi[4], (float)Mi[5], (float)Mi[6], (float)Mi[7]);
-    f4[2] = make_float4((float)Mi[8], (float)wd, 0.f, 0.f);
+    geometry_store(geo + ((size_t)e * np + p) * kGeoStride, Mi, wd);
   }
 }
 
+// Affine: thread per element.  Curved: thread per (element, point), so
+// high-order elements with many points keep the device busy.
 __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
                            const double* __restrict__ xyz, const int32_t* __restrict__ conn,
                            const double* __restrict__ dN, const double* __restrict__ w,
                            double* __restrict__ geo) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  geometry_one(E, N, npe, q, affine, xyz, conn, dN, w, geo, e);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (affine) {
+    if (t >= E) return;
+    geometry_one(E, N, npe, q, 1, xyz, conn, dN, w, geo, t);
+    return;
+  }
+  if (t >= E * q) return;
+  const int64_t e = t / q;
+  const int p = (int)(t - e * q);
+  double J[9], Mi[9];
+  jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
+  const double det = inv3(J, Mi);
+  geometry_store(geo + (size_t)t * kGeoStride, Mi, fabs(det) * w[p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,10 +368,12 @@ __device__ __forceinline__ StepRec load_rec(const PanelParams& P, int64_t r, int
   return o;
 }
 
-// One wavefront's share of a panel launch: wave w of the launch, LDS panel
-// space wl (P.lds_per_wave doubles).  Uses no block-level barrier.
+// Waves [w0, w0 + nw) of a panel launch run by one wavefront, LDS panel space
+// wl (P.lds_per_wave doubles).  Consecutive waves' records and node
+// descriptors are contiguous, so a span is one longer stream of nw * rounds
+// rounds.  Uses no block-level barrier.
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
-__device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w, double* wl) {
+__device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int nw, double* wl) {
   constexpr int NPW = 64 / NPE;
   constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
   const int lane = threadIdx.x & 63;
@@ -364,9 +381,9 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w, doub
   const int n = lane / NPE;
   const int b = lane - n * NPE;
   const bool grp = n < NPW;
-  const int G = P.rounds, cnt = P.cnt;
+  const int G = P.rounds * nw, cnt = P.cnt;
   const int64_t nsteps = (int64_t)G * cnt;
-  const int64_t rbase = w * nsteps * NPW;         // first record of this wave
+  const int64_t rbase = w0 * P.rounds * (int64_t)cnt * NPW;   // first record of the span
   const int64_t rlast = rbase + nsteps * NPW - 1;
   const int64_t lane_rec = grp ? n : 0;
   double* const pan = wl + (size_t)(grp ? n : 0) * P.max_r * BLK;
@@ -374,7 +391,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w, doub
   for (int i = lane; i < NPW * P.max_r * BLK + 9; i += 64) wl[i] = 0.0;
 
   // node descriptors of the current and next round
-  const int2* desc = P.desc + w * (int64_t)G * NPW;
+  const int2* desc = P.desc + w0 * (int64_t)P.rounds * NPW;
   int2 dcur = desc[lane_rec];
   int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
 
@@ -462,7 +479,7 @@ __global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
   const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;   // logical block
   const int64_t w = lb * (blockDim.x >> 6) + wave;
   if (w >= P.waves) return;                       // no block-level barriers below
-  panel_wave<NPE, MapT, OP, AFFINE, ISO, Real>(P, w, lds + (size_t)wave * P.lds_per_wave);
+  panel_wave<NPE, MapT, OP, AFFINE, ISO, Real>(P, w, 1, lds + (size_t)wave * P.lds_per_wave);
 }
 
 // lumped mass: row sums of the consistent mass panels (src/linear.jl:1999-2001)

@@ -566,6 +566,72 @@ __global__ void k_i64_sub(int64_t n, int64_t* __restrict__ a, int64_t base) {
 
 }  // namespace
 
+// Upload f and the fixed set, run one system through the batched engine,
+// download u.
+int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const double* f, const int64_t* fixed,
+                        int64_t nfixed, int32_t one_based, const double* u_fixed, double* u_out,
+                        llfem_stats* stats) {
+  hipStream_t st = ctx->stream;
+  const int64_t n = 3 * sp.N;
+  const size_t bytes = sizeof(double) * 3 * (size_t)n + (size_t)n + sizeof(int64_t) * (size_t)(nfixed + 1) +
+                       sizeof(double) * (size_t)(nfixed + 1) + 64;
+  char* base = nullptr;
+  LLFEM_TRY(dmalloc(ctx, (void**)&base, bytes));
+  struct Free {
+    llfem_ctx* c; void* p; size_t b;
+    ~Free() { (void)hipStreamSynchronize(c->stream); dfree(c, p, b); }
+  } guard{ctx, base, bytes};
+  double* d_f = (double*)base;
+  double* d_u = d_f + n;
+  double* d_ufix = d_u + n;
+  int64_t* d_idx = (int64_t*)(d_ufix + n);
+  double* d_vals = (double*)(d_idx + nfixed + 1);
+  int* d_err = (int*)(d_vals + nfixed + 1);
+  uint8_t* d_mask = (uint8_t*)(d_err + 2);
+  LLFEM_HIP(hipMemcpyAsync(d_f, f, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  LLFEM_HIP(hipMemsetAsync(d_mask, 0, (size_t)n, st));
+  LLFEM_HIP(hipMemsetAsync(d_ufix, 0, sizeof(double) * n, st));
+  LLFEM_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
+  if (nfixed > 0) {
+    LLFEM_HIP(hipMemcpyAsync(d_idx, fixed, sizeof(int64_t) * nfixed, hipMemcpyHostToDevice, st));
+    if (u_fixed)
+      LLFEM_HIP(hipMemcpyAsync(d_vals, u_fixed, sizeof(double) * nfixed, hipMemcpyHostToDevice, st));
+    k_set_fixed<<<(unsigned)((nfixed + 255) / 256), 256, 0, st>>>(nfixed, d_idx, one_based ? 1 : 0, n,
+                                                                   u_fixed ? d_vals : nullptr, d_mask,
+                                                                   d_ufix, d_err);
+  }
+  int herr = 0;
+  LLFEM_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  if (herr) return fail(LLFEM_EINVAL, "solveDisplacement: constrained DoF out of range");
+  auto t0 = std::chrono::steady_clock::now();
+  sp.fixed = d_mask;
+  sp.f = d_f;
+  sp.ufix = d_ufix;
+  BatchSolver eng(ctx, f64_vectors);
+  SolveSys* s = nullptr;
+  LLFEM_TRY(eng.add(sp, &s));
+  while (sys_status(s) == 0) {
+    std::vector<SolveSys*> done;
+    LLFEM_TRY(eng.run(32, &done));
+  }
+  LLFEM_TRY(eng.finish(s, d_u));
+  LLFEM_HIP(hipMemcpyAsync(u_out, d_u, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  const int status = sys_status(s);
+  if (stats) {
+    stats->iterations = sys_iterations(s);
+    stats->refinements = sys_updates(s);
+    stats->relres = sys_relres(s);
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const double rr = sys_relres(s);
+  eng.remove(s);
+  if (status != 1)
+    return fail(LLFEM_ENOCONV, "cg: maxiter reached (relative residual " + std::to_string(rr) + ")");
+  return LLFEM_OK;
+}
+
 // Make sure the mesh's FP32 copy of K exists (mixed-precision solve).
 int ensure_k32(llfem_mesh* m) {
   if (m->d_valK32 && m->solver_ready) return LLFEM_OK;
@@ -588,10 +654,24 @@ int llfem_solve(llfem_mesh* m, const double* f, const int64_t* fixed, int64_t nf
                 llfem_stats* stats) {
   if (!m || !f || !u_out || (nfixed > 0 && !fixed)) return fail(LLFEM_EINVAL, "llfem_solve: NULL argument");
   if (!m->have_K) return fail(LLFEM_EINVAL, "llfem_solve: no stiffness assembled on this mesh");
-  if (method != LLFEM_PCG_JACOBI_FP64 && method != LLFEM_PCG_MIXED_REFINE)
+  if (method != LLFEM_PCG_JACOBI_FP64 && method != LLFEM_PCG_MIXED_REFINE &&
+      method != LLFEM_PCG_MIXED_RELIABLE && method != LLFEM_PCG_F32OP_RELIABLE)
     return fail(LLFEM_EUNSUP, "llfem_solve: unknown method");
   if (!(reltol > 0)) return fail(LLFEM_EINVAL, "llfem_solve: reltol must be positive");
   LLFEM_HIP(hipSetDevice(m->ctx->device));
+  if (method == LLFEM_PCG_MIXED_RELIABLE || method == LLFEM_PCG_F32OP_RELIABLE) {
+    SysSpec sp;
+    sp.N = m->N;
+    sp.nadj = m->nadj;
+    sp.adj_ptr = m->d_adj_ptr;
+    sp.adj = m->d_adj;
+    sp.dslot = m->d_diag_slot;
+    sp.val64 = m->d_valK;
+    sp.reltol = reltol;
+    sp.maxiter = maxiter;
+    return solve_host_reliable(m->ctx, sp, method == LLFEM_PCG_F32OP_RELIABLE, f, fixed, nfixed, one_based,
+                               u_fixed, u_out, stats);
+  }
   if (method == LLFEM_PCG_MIXED_REFINE) LLFEM_TRY(ensure_k32(m));
   PanelOp K{m->N, m->d_adj_ptr, m->d_adj, m->d_diag_slot};
   return solve_host(m->ctx, K, m->d_valK, m->d_valK32, 3 * m->N, f, fixed, nfixed, one_based, u_fixed,

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libllfem.so")
 OK, EINVAL, EHIP, ENOMEM, EGEOM, EPATTERN, ENOCONV, EUNSUP = 0, -1, -2, -3, -4, -5, -6, -7
 OP_STIFFNESS, OP_MASS = 0, 1
 FP64, FP32_INT_FP64_SUM, BF16X3 = 0, 1, 2
-PCG_JACOBI_FP64, PCG_MIXED_REFINE = 0, 1
+PCG_JACOBI_FP64, PCG_MIXED_REFINE, PCG_MIXED_RELIABLE, PCG_F32OP_RELIABLE = 0, 1, 2, 3
 
 EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",

@@ -151,6 +151,41 @@ def test_solve_golden_sum_u_30(gpu_ctx):
     say(f"  mixed: {sm.iterations} it, {sm.refinements} refinements, relres {sm.relres:.2e}")
     assert sm.relres < 1e-10
     assert abs(um.sum() / 0.07057241535937334 - 1) < 1e-8
+    ur, sr = dev.solve(f, fixed, method=_lib.PCG_MIXED_RELIABLE, reltol=1e-10)
+    say(f"  mixed reliable-update: {sr.iterations} it, {sr.refinements} updates, relres {sr.relres:.2e}")
+    assert sr.relres < 1e-10
+    assert abs(ur.sum() / 0.07057241535937334 - 1) < 1e-8
+    assert np.all(ur[fixed] == 0)
+    uo, so = dev.solve(f, fixed, method=_lib.PCG_F32OP_RELIABLE, reltol=1e-10)
+    say(f"  FP32 operator / FP64 vectors: {so.iterations} it, {so.refinements} updates, relres {so.relres:.2e}")
+    assert so.relres < 1e-10
+    assert abs(uo.sum() / 0.07057241535937334 - 1) < 1e-8
+
+
+def test_solve_reliable_inhomogeneous_dirichlet(gpu_ctx):
+    """Mixed-precision reliable-update PCG on the mesh path with prescribed
+    non-zero displacements, against the oracle's FP64 PCG."""
+    m = M.structured_box_mesh(n=4, elem="tet", order=2)
+    tab = R.rule_tables(11, 2)
+    cp, rv = O.csc_pattern(m)
+    K = sp.csc_matrix((O.stiffness_csc(m, tab, hooke_symgrad_D(), cp, rv), rv, cp))
+    dev = _dev(gpu_ctx, m)
+    dev.assemble(_lib.OP_STIFFNESS, tab, copy_out=False)
+    f = np.zeros(3 * m.N)
+    left, right = m.nodes_on("left"), m.nodes_on("right")
+    fixed = np.concatenate([3 * left + c for c in range(3)] + [3 * right])
+    ufix = np.concatenate([np.zeros(3 * left.size), np.full(right.size, 1e-3)])
+    u, st = dev.solve(f, fixed, ufix, method=_lib.PCG_MIXED_RELIABLE, reltol=1e-10)
+    u3, _ = dev.solve(f, fixed, ufix, method=_lib.PCG_F32OP_RELIABLE, reltol=1e-10)
+    uo, it, rel = O.pcg(K, f, fixed, ufix, reltol=1e-12)
+    assert np.linalg.norm(u3 - uo) / np.linalg.norm(uo) < 1e-8
+    say(f"  reliable: {st.iterations} it, {st.refinements} updates; oracle FP64 {it} it")
+    assert np.all(u[3 * left] == 0) and np.abs(u[3 * right] - 1e-3).max() == 0
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+    free = np.setdiff1d(np.arange(3 * m.N), fixed)
+    fk = K[:, fixed] @ u[fixed]
+    r = K @ u - f
+    assert np.linalg.norm(r[free]) / np.linalg.norm(fk[free]) < 1e-10
 
 
 def test_solve_csc_and_dirichlet_values(gpu_ctx):
