@@ -41,7 +41,11 @@ namespace {
 constexpr int kSB = 256;                 // threads per block
 constexpr int kSG = 8;                   // lanes per node row in the SpMV
 constexpr int kSpmvNodes = kSB / kSG;    // nodes per SpMV block
-constexpr int kVecNodes = kSB;           // nodes per vector block
+constexpr int kVecNodes = kSB;           // nodes per vector block pass
+// Blocks per system are capped (the blocks stride over the system's nodes),
+// so the per-system reductions sum at most a few thousand partials.
+constexpr int64_t kMaxSpmvBlocks = 2048;
+constexpr int64_t kMaxVecBlocks = 1024;
 
 enum { SC_RHO = 0, SC_ALPHA, SC_BETA, SC_RR, SC_MAXRR, SC_TOL2, SC_DELTA2, SC_STATE, SC_ITER, SC_PQ, SC_N = 16 };
 enum { ST_RUN = 0, ST_PENDING = 1, ST_PAUSED = 2, ST_IDLE = 3 };
@@ -81,13 +85,16 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
   const VT* __restrict__ pv = static_cast<const VT*>(S.p);
   VT* __restrict__ qv = static_cast<VT*>(S.q);
   const int lb = blockIdx.x - S.sblk0;
-  const int64_t j = (int64_t)lb * kSpmvNodes + threadIdx.x / kSG;
   const int l = threadIdx.x % kSG;
+  double d = 0.0;
+  for (int64_t j = (int64_t)lb * kSpmvNodes + threadIdx.x / kSG; j - threadIdx.x / kSG < S.N;
+       j += (int64_t)S.nsblk * kSpmvNodes) {
   VT a0 = 0, a1 = 0, a2 = 0;
   if (j < S.N) {
     const int64_t p0 = S.adj_ptr[j];
     const int R = (int)(S.adj_ptr[j + 1] - p0);
     const float* P = S.val + 9 * p0;
+#pragma unroll 2
     for (int t = l; t < R; t += kSG) {
       const int64_t i = S.adj[p0 + t];
       const VT x0 = pv[3 * i], x1 = pv[3 * i + 1], x2 = pv[3 * i + 2];
@@ -103,7 +110,6 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
     a1 += __shfl_xor(a1, o, kSG);
     a2 += __shfl_xor(a2, o, kSG);
   }
-  double d = 0.0;
   if (j < S.N && l == 0) {
     const VT a[3] = {a0, a1, a2};
 #pragma unroll
@@ -112,6 +118,7 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
       qv[3 * j + c] = v;
       d += (double)pv[3 * j + c] * (double)v;
     }
+  }
   }
   d = bsum(d, sh);
   if (threadIdx.x == 0) S.part[lb] = d;
@@ -154,10 +161,9 @@ __global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sy
   const VT* __restrict__ pv = static_cast<const VT*>(S.p);
   const VT* __restrict__ qv = static_cast<const VT*>(S.q);
   const int lb = blockIdx.x - S.vblk0;
-  const int64_t j = (int64_t)lb * kVecNodes + threadIdx.x;
   double rr = 0.0, rz = 0.0;
-  if (j < S.N) {
-    const VT alpha = (VT)S.scal[SC_ALPHA];
+  const VT alpha = (VT)S.scal[SC_ALPHA];
+  for (int64_t j = (int64_t)lb * kVecNodes + threadIdx.x; j < S.N; j += (int64_t)S.nvblk * kVecNodes) {
     VT r[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -215,11 +221,10 @@ __global__ __launch_bounds__(kSB) void k_bs_pupd(const SysDev* __restrict__ sys,
   VT* __restrict__ pv = static_cast<VT*>(S.p);
   const VT* __restrict__ zv = static_cast<const VT*>(S.z);
   const int lb = blockIdx.x - S.vblk0;
-  const int64_t j = (int64_t)lb * kVecNodes + threadIdx.x;
-  if (j >= S.N) return;
   const VT beta = (VT)S.scal[SC_BETA];
+  for (int64_t j = (int64_t)lb * kVecNodes + threadIdx.x; j < S.N; j += (int64_t)S.nvblk * kVecNodes)
 #pragma unroll
-  for (int c = 0; c < 3; ++c) pv[3 * j + c] = zv[3 * j + c] + beta * pv[3 * j + c];
+    for (int c = 0; c < 3; ++c) pv[3 * j + c] = zv[3 * j + c] + beta * pv[3 * j + c];
 }
 
 __global__ void k_bs_states(const SysDev* __restrict__ sys, int nsys, double* __restrict__ out) {
@@ -395,6 +400,12 @@ __global__ void k_fill_u(int64_t n, const uint8_t* __restrict__ fixed, const dou
 }
 
 inline unsigned blocks(int64_t n, int bs = kSB) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+inline int64_t spmv_blocks(int64_t N) {
+  return std::max<int64_t>(1, std::min<int64_t>((N + kSpmvNodes - 1) / kSpmvNodes, kMaxSpmvBlocks));
+}
+inline int64_t vec_blocks(int64_t N) {
+  return std::max<int64_t>(1, std::min<int64_t>((N + kVecNodes - 1) / kVecNodes, kMaxVecBlocks));
+}
 
 }  // namespace
 
@@ -431,7 +442,7 @@ void BatchSolver::release(SolveSys* s) {
 
 size_t BatchSolver::bytes_needed(int64_t N, int64_t nadj, bool f64_vectors) {
   const int64_t n = 3 * N;
-  const int64_t nsblk = (N + kSpmvNodes - 1) / kSpmvNodes, nvblk = (N + kVecNodes - 1) / kVecNodes;
+  const int64_t nsblk = spmv_blocks(N), nvblk = vec_blocks(N);
   const int64_t npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
   return sizeof(double) * (4 * (size_t)n + SC_N + (size_t)npart) +
          (f64_vectors ? sizeof(double) : sizeof(float)) * 5 * (size_t)n +
@@ -505,7 +516,7 @@ int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
     return rc;
   }
   const int64_t n = s->n, N = spec.N;
-  const int64_t nsblk = (N + kSpmvNodes - 1) / kSpmvNodes, nvblk = (N + kVecNodes - 1) / kVecNodes;
+  const int64_t nsblk = spmv_blocks(N), nvblk = vec_blocks(N);
   s->npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
   char* m = static_cast<char*>(s->mem);
   auto take = [&](size_t bytes) -> char* {
@@ -569,8 +580,8 @@ int BatchSolver::rebuild_tables() {
     d.x = s->x; d.r = s->r; d.z = s->z; d.p = s->p; d.q = s->q;
     d.scal = s->scal;
     d.part = s->part;
-    d.nsblk = (int32_t)((sp.N + kSpmvNodes - 1) / kSpmvNodes);
-    d.nvblk = (int32_t)((sp.N + kVecNodes - 1) / kVecNodes);
+    d.nsblk = (int32_t)spmv_blocks(sp.N);
+    d.nvblk = (int32_t)vec_blocks(sp.N);
     d.sblk0 = (int32_t)sb.size();
     d.vblk0 = (int32_t)vb.size();
     const int32_t id = (int32_t)dev.size();
