@@ -207,6 +207,66 @@ typedef struct {
 
 LLFEM_API int llfem_autotune(llfem_mesh* sample, const llfem_op_desc* desc, llfem_tune_result* out);
 
+/* ---- serving -------------------------------------------------------------
+ * A device-resident loop for assemble-and-solve requests, each the
+ * reference's  K = stiffnessMatrix(problem); f = loadVector(problem,
+ * [load(face, fx, fy, fz)]); u = solveDisplacement(K, f, support = [...])
+ * (src/linear.jl:70-87, 3088-3434, 3711-3743) on its own mesh and material.
+ * A worker thread (own context on `device`) admits requests first come first
+ * served while fewer than max_active run and their allocations fit the HBM
+ * budget (0 = no budget), and advances all admitted solves together, one
+ * launch per kernel for the whole batch (continuous batching).  Meshes with
+ * their CSC pattern and element->nnz map, face-load weights and
+ * constrained-DoF masks stay on the device for later requests with the same
+ * content (or the same non-zero mesh_key).  A request's result does not
+ * depend on the other requests in flight.
+ * Every array a request points to must stay valid until llfem_server_wait
+ * returns for it; u_out is written before the request completes. */
+typedef struct llfem_server llfem_server;
+
+typedef struct {
+  const double* xyz;          /* [N][3] */
+  int64_t N;
+  int32_t elem_type, npe;     /* gmsh volume element type */
+  const int64_t* conn;        /* [E][npe], 0-based */
+  int64_t E;
+  llfem_op_desc stiffness;    /* volume rule, E, nu (op is taken as stiffness) */
+  int32_t face_type, nf;      /* traction face set (gmsh face type), 0-based [F][nf] */
+  const int64_t* face_conn;
+  int64_t F;
+  int32_t face_q;             /* face rule: q, weights [q], N [q][nf], dN [q][nf][3] */
+  const double* face_weights;
+  const double* face_N_ref;
+  const double* face_dN_ref;
+  double traction[3];         /* constant traction on the face set */
+  const int64_t* fixed;       /* constrained DoFs (0-based), held at 0 */
+  int64_t nfixed;
+  int32_t method;             /* LLFEM_PCG_MIXED_RELIABLE or LLFEM_PCG_F32OP_RELIABLE */
+  double reltol;              /* ||b - K x|| <= reltol ||b|| */
+  int64_t maxiter;
+  uint64_t mesh_key;          /* 0: key the mesh cache by content */
+  double* u_out;              /* [3N] */
+} llfem_request;
+
+typedef struct {
+  double latency_ms;          /* submit -> result written */
+  double queue_ms;            /* submit -> admitted */
+  double service_ms;          /* admitted -> result written */
+  int64_t iterations;         /* PCG iterations */
+  int64_t updates;            /* FP64 reliable updates */
+  double relres;              /* ||(K u - f)_free|| / ||f_free|| */
+  int32_t mesh_cache_hit;
+  int32_t status;             /* LLFEM_OK or the request's error code */
+} llfem_request_stats;
+
+LLFEM_API int llfem_server_create(int device, size_t hbm_budget_bytes, int32_t max_active, llfem_server** out);
+LLFEM_API int llfem_server_submit(llfem_server* server, const llfem_request* request, int64_t* id);
+/* Blocks until request `id` completes; returns its status (message in
+ * llfem_last_error on failure).  Each id is waited for once. */
+LLFEM_API int llfem_server_wait(llfem_server* server, int64_t id, llfem_request_stats* stats);
+/* Finishes the requests already submitted, then frees everything. */
+LLFEM_API int llfem_server_destroy(llfem_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@ EXPORTED = [
     "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
     "llfem_assemble", "llfem_load_traction", "llfem_solve", "llfem_solve_csc",
     "llfem_assemble_batch", "llfem_mesh_values", "llfem_autotune",
+    "llfem_server_create", "llfem_server_submit", "llfem_server_wait", "llfem_server_destroy",
 ]
 
 
@@ -47,6 +48,22 @@ class Stats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("elem_type", C.c_int32), ("mult", C.c_int32), ("waves_per_block", C.c_int32),
                 ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
+class RequestSpec(C.Structure):   # llfem_request
+    _fields_ = [("xyz", C.c_void_p), ("N", C.c_int64), ("elem_type", C.c_int32), ("npe", C.c_int32),
+                ("conn", C.c_void_p), ("E", C.c_int64), ("stiffness", OpDesc),
+                ("face_type", C.c_int32), ("nf", C.c_int32), ("face_conn", C.c_void_p), ("F", C.c_int64),
+                ("face_q", C.c_int32), ("face_weights", C.c_void_p), ("face_N_ref", C.c_void_p),
+                ("face_dN_ref", C.c_void_p), ("traction", C.c_double * 3), ("fixed", C.c_void_p),
+                ("nfixed", C.c_int64), ("method", C.c_int32), ("reltol", C.c_double),
+                ("maxiter", C.c_int64), ("mesh_key", C.c_uint64), ("u_out", C.c_void_p)]
+
+
+class RequestStats(C.Structure):  # llfem_request_stats
+    _fields_ = [("latency_ms", C.c_double), ("queue_ms", C.c_double), ("service_ms", C.c_double),
+                ("iterations", C.c_int64), ("updates", C.c_int64), ("relres", C.c_double),
+                ("mesh_cache_hit", C.c_int32), ("status", C.c_int32)]
 
 
 class LLFEMError(RuntimeError):
@@ -84,6 +101,10 @@ def load_library(path: str = LIB_PATH):
     lib.llfem_assemble_batch.argtypes = [I32, P, P, C.POINTER(Stats)]
     lib.llfem_mesh_values.argtypes = [P, I32, I32, P]
     lib.llfem_autotune.argtypes = [P, C.POINTER(OpDesc), C.POINTER(TuneResult)]
+    lib.llfem_server_create.argtypes = [C.c_int, SZ, I32, C.POINTER(P)]
+    lib.llfem_server_submit.argtypes = [P, C.POINTER(RequestSpec), C.POINTER(I64)]
+    lib.llfem_server_wait.argtypes = [P, I64, C.POINTER(RequestStats)]
+    lib.llfem_server_destroy.argtypes = [P]
     for name in EXPORTED:
         fn = getattr(lib, name)
         if name not in ("llfem_last_error", "llfem_ctx_bytes_in_use"):
@@ -269,3 +290,61 @@ def solve_csc(ctx: Context, colptr, rowval, nzval, f, fixed, ufix=None, method=P
                                   fx.size, ptr(uf), method, float(reltol), int(maxiter), ptr(u),
                                   C.byref(st)))
     return u, st
+
+
+class Server:
+    """Assemble-and-solve server (llfem_server): continuous batching of
+    requests on one device, meshes and their patterns cached across requests."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 64):
+        self.lib = load_library()
+        h = C.c_void_p()
+        check(self.lib.llfem_server_create(device, hbm_budget, max_active, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, mesh, tables, face_tables, face, traction, fixed, E=2e5, nu=0.3,
+               method=PCG_MIXED_RELIABLE, reltol=1e-10, maxiter=200000, mesh_key=0):
+        """mesh: lowlevelfem.mesh.Mesh; face: a FaceSet of it; fixed: 0-based DoFs."""
+        xyz = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+        cn = np.ascontiguousarray(mesh.conn, dtype=np.int64)
+        fc = np.ascontiguousarray(face.conn, dtype=np.int64)
+        fx = np.ascontiguousarray(fixed, dtype=np.int64)
+        fw = np.ascontiguousarray(face_tables.weights, dtype=np.float64)
+        fN = np.ascontiguousarray(face_tables.N, dtype=np.float64)
+        fdN = np.ascontiguousarray(face_tables.dN, dtype=np.float64)
+        desc, keep_desc = make_desc(OP_STIFFNESS, tables, E, nu)
+        u = np.empty(3 * xyz.shape[0], dtype=np.float64)
+        r = RequestSpec()
+        r.xyz, r.N, r.elem_type, r.npe = ptr(xyz).value, xyz.shape[0], int(mesh.etype), cn.shape[1]
+        r.conn, r.E, r.stiffness = ptr(cn).value, cn.shape[0], desc
+        r.face_type, r.nf, r.face_conn, r.F = int(face.etype), fc.shape[1], ptr(fc).value, fc.shape[0]
+        r.face_q, r.face_weights = face_tables.q, ptr(fw).value
+        r.face_N_ref, r.face_dN_ref = ptr(fN).value, ptr(fdN).value
+        r.traction = (C.c_double * 3)(*[float(t) for t in traction])
+        r.fixed, r.nfixed = (ptr(fx).value if fx.size else None), fx.size
+        r.method, r.reltol, r.maxiter, r.mesh_key = method, float(reltol), int(maxiter), int(mesh_key)
+        r.u_out = ptr(u).value
+        rid = C.c_int64()
+        check(self.lib.llfem_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = (xyz, cn, fc, fx, fw, fN, fdN, keep_desc, u, r)
+        return rid.value
+
+    def wait(self, rid):
+        st = RequestStats()
+        code = self.lib.llfem_server_wait(self.h, int(rid), C.byref(st))
+        keep = self._keep.pop(int(rid))
+        check(code)
+        return keep[8], st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.llfem_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
