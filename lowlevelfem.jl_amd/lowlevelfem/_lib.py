@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libllfem.so")
+LIB_PATH = os.environ.get("LLFEM_LIB", os.path.join(_HERE, "libllfem.so"))
 
 OK, EINVAL, EHIP, ENOMEM, EGEOM, EPATTERN, ENOCONV, EUNSUP = 0, -1, -2, -3, -4, -5, -6, -7
 OP_STIFFNESS, OP_MASS = 0, 1
