@@ -82,6 +82,22 @@ def test_golden_corner_entries_40():
     assert cp[-1] == 15_944_049                  # structural nnz (DSL_vs_legacy.ipynb:736)
 
 
+def test_golden_corner_entries_hex27_20():
+    """Order-2 hexahedra: K[1,1], K[2,1], K[3,1] = 877.4928774928592,
+    320.51282051281675, -320.51282051281373 at 20³ with Gauss5
+    (examples/dev/multi-thread.ipynb:75), at the same gmsh corner (0,0,1)."""
+    m = M.structured_box_mesh(n=20, order=2)
+    tab = R.rule_tables(12, 5)
+    cp, rv = O.csc_pattern(m)
+    nz = O.stiffness_csc(m, tab, hooke_symgrad_D(), cp, rv, nthreads=4)
+    j = 41 * 41 * 40
+    col = slice(cp[3 * j], cp[3 * j + 1])
+    vals = dict(zip(rv[col], nz[col]))
+    ref = [877.4928774928592, 320.51282051281675, -320.51282051281373]
+    for c in range(3):
+        assert abs(vals[3 * j + c] / ref[c] - 1) < 1e-12
+
+
 def test_golden_sum_u_30():
     """sum(u) = 0.07057241535937334: left clamped, traction (1,0,0) on
     right, 30³ hex8 (results.txt:7; Ferrite gives ...37218)."""
