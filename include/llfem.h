@@ -61,7 +61,10 @@ enum {
   LLFEM_PCG_MIXED_RELIABLE = 2, /* FP32 block-Jacobi PCG with FP64 reliable updates (the true
                                    FP64 residual replaces the recursive one every 10x reduction;
                                    converged on ||b - K x|| <= reltol ||b||); llfem_solve only */
-  LLFEM_PCG_F32OP_RELIABLE = 3  /* as 2 with FP64 Krylov vectors: only the operator is FP32 */
+  LLFEM_PCG_F32OP_RELIABLE = 3, /* as 2 with FP64 Krylov vectors: only the operator is FP32 */
+  LLFEM_PCG_AMG_RELIABLE = 4    /* as 2 with an aggregation-AMG V-cycle as the preconditioner
+                                   (rigid-body-mode coarse spaces, Chebyshev smoothing); the
+                                   hierarchy is built once per mesh and support and kept */
 };
 
 /* One operator request.  The tables are what gmsh hands the reference:
@@ -93,6 +96,8 @@ typedef struct {
   int64_t refinements;     /* solver: FP64 refinement steps */
   double relres;           /* solver: ||(K u - f)_free|| / ||f_free|| */
   double alg_flops;        /* element-integration flops of the call (see llfem_assemble) */
+  double setup_ms;         /* solver: preconditioner set-up wall time (AMG: hierarchy build when
+                              not cached, plus the per-solve coarse operators) */
 } llfem_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -241,7 +246,7 @@ typedef struct {
   double traction[3];         /* constant traction on the face set */
   const int64_t* fixed;       /* constrained DoFs (0-based), held at 0 */
   int64_t nfixed;
-  int32_t method;             /* LLFEM_PCG_MIXED_RELIABLE or LLFEM_PCG_F32OP_RELIABLE */
+  int32_t method;             /* LLFEM_PCG_MIXED_RELIABLE, _F32OP_RELIABLE or _AMG_RELIABLE */
   double reltol;              /* ||b - K x|| <= reltol ||b|| */
   int64_t maxiter;
   uint64_t mesh_key;          /* 0: key the mesh cache by content */

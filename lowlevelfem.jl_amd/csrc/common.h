@@ -67,6 +67,8 @@ struct Tune {
   double ms = 0.0;
 };
 
+struct AmgHier;
+
 // A device copy of a reference table, shared by every mesh of a context.
 struct DevTable {
   double* ptr = nullptr;
@@ -142,6 +144,9 @@ struct llfem_mesh {
   int32_t* d_blk_node = nullptr;  // [nadj] column node of each nnz block
   void* d_ho = nullptr;           // staged blocks + Z
   size_t ho_bytes = 0;
+  // AMG hierarchies kept per support (key: hash of the constrained-DoF list),
+  // each with the device mask it was built for
+  std::map<uint64_t, std::pair<llfem::AmgHier*, uint8_t*>> amg;
 };
 
 namespace llfem {
@@ -196,6 +201,7 @@ struct SysSpec {
   const double* ufix = nullptr;     // [3N] prescribed values (NULL = 0)
   double reltol = 1e-10;
   int64_t maxiter = 100000;
+  const struct AmgHier* amg = nullptr;   // AMG preconditioner (FP32 vectors only); NULL = Jacobi
 };
 
 struct SolveSys;
@@ -224,12 +230,42 @@ class BatchSolver {
   void* tables_ = nullptr;
   size_t tables_bytes_ = 0;
   void* d_sys_ = nullptr;
+  struct AmgBatch* amgb_ = nullptr;
   int32_t* d_sb_ = nullptr;
   int32_t* d_vb_ = nullptr;
   double* d_states_ = nullptr;
   int nsys_ = 0;
   int64_t nsb_ = 0, nvb_ = 0;
 };
+
+// ---- aggregation AMG preconditioner (amg.hip) ------------------------------------
+struct AmgInput;   // pattern, strong couplings, coordinates, support (host copies)
+struct AmgHost;    // host-built hierarchy
+struct AmgHier;    // device hierarchy of one mesh + support
+struct AmgSys;     // numeric levels of one system
+struct AmgBatch;   // V-cycle launch tables over several systems
+int amg_gather(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgInput* in);
+int amg_build_host(const AmgInput& in, AmgHost* h);
+int amg_upload(llfem_ctx* ctx, const AmgHost& H, const SysSpec& sp, AmgHier** out);
+void amg_free(llfem_ctx* ctx, AmgHier* h);
+int64_t amg_levels(const AmgHier* h);
+size_t amg_bytes(const AmgHier* h);
+double amg_host_ms(const AmgHier* h);
+int amg_sys_create(llfem_ctx* ctx, const AmgHier* h, const float* val32, float* r, float* z, float* q,
+                   double* rzpart, AmgSys** out);
+void amg_sys_free(llfem_ctx* ctx, AmgSys* s);
+int amg_apply_solo(llfem_ctx* ctx, AmgSys* s);
+int amg_batch_build(llfem_ctx* ctx, const std::vector<AmgSys*>& sys, const std::vector<const double*>& states,
+                    AmgBatch** b);
+int amg_batch_apply(llfem_ctx* ctx, AmgBatch* b);
+void amg_batch_free(llfem_ctx* ctx, AmgBatch* b);
+// One-call hierarchy build for a system (gather + host build + upload).
+int amg_build(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgHier** out);
+// The same with the host build on a thread of its own.
+struct AmgJob;
+int amg_job_start(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgJob** job);
+bool amg_job_ready(AmgJob* job);
+int amg_job_finish(llfem_ctx* ctx, AmgJob* job, const SysSpec& sp, AmgHier** out);   // frees the job
 
 int64_t sys_iterations(const SolveSys* s);
 int64_t sys_updates(const SolveSys* s);

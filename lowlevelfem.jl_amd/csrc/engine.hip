@@ -61,6 +61,7 @@ struct SysDev {
   double* scal;            // [SC_N]
   double* part;            // [nsblk] SpMV partials, then [2 * nvblk]
   int32_t sblk0, nsblk, vblk0, nvblk;
+  int32_t amg;             // 1: z and the r.z partials come from the AMG V-cycle
 };
 
 __device__ __forceinline__ double bsum(double v, double* sh) {
@@ -172,12 +173,14 @@ __global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sy
       r[c] = rv[i] - alpha * qv[i];
       rv[i] = r[c];
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rr += (double)r[c] * (double)r[c];
+    if (S.amg) continue;
     const float* B = S.binv + 9 * j;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const VT z = (VT)B[3 * c] * r[0] + (VT)B[3 * c + 1] * r[1] + (VT)B[3 * c + 2] * r[2];
       zv[3 * j + c] = z;
-      rr += (double)r[c] * (double)r[c];
       rz += (double)r[c] * (double)z;
     }
   }
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sy
   rz = bsum(rz, sh);
   if (threadIdx.x == 0) {
     S.part[S.nsblk + 2 * lb] = rr;
-    S.part[S.nsblk + 2 * lb + 1] = rz;
+    if (!S.amg) S.part[S.nsblk + 2 * lb + 1] = rz;
   }
 }
 
@@ -356,6 +359,51 @@ __global__ __launch_bounds__(kSB) void k_ru_scal(int64_t nb, const double* __res
   }
 }
 
+// AMG restart, first half: r = rh * s; p = pscale * p (not on the first
+// start); partial r.r per engine vector block (the V-cycle adds r.z).
+__global__ __launch_bounds__(kSB) void k_ru_restart_amg(int64_t N, int32_t nvblk, const double* __restrict__ rh,
+                                                        double s, double pscale, int init, float* __restrict__ r32,
+                                                        float* __restrict__ p32, double* __restrict__ part) {
+  __shared__ double sh[kSB];
+  double rr = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kVecNodes + threadIdx.x; j < N; j += (int64_t)nvblk * kVecNodes)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float r = (float)(rh[3 * j + c] * s);
+      r32[3 * j + c] = r;
+      if (!init) p32[3 * j + c] = (float)pscale * p32[3 * j + c];
+      rr += (double)r * (double)r;
+    }
+  rr = bsum(rr, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = rr;
+}
+
+// AMG restart, second half: p = z on the first start; scalars from the
+// partials (one block).
+__global__ __launch_bounds__(kSB) void k_ru_scal_amg(int64_t n, int32_t nvblk, int init, const float* __restrict__ z,
+                                                     float* __restrict__ p, const double* __restrict__ part,
+                                                     double* __restrict__ scal, double tol2, double delta2) {
+  __shared__ double sh[kSB];
+  if (init)
+    for (int64_t i = threadIdx.x; i < n; i += kSB) p[i] = z[i];
+  double rr = 0.0, rz = 0.0;
+  for (int i = threadIdx.x; i < nvblk; i += kSB) {
+    rr += part[2 * i];
+    rz += part[2 * i + 1];
+  }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    scal[SC_RHO] = rz;
+    scal[SC_RR] = rr;
+    scal[SC_MAXRR] = rr;
+    scal[SC_TOL2] = tol2;
+    scal[SC_DELTA2] = delta2;
+    scal[SC_BETA] = 0.0;
+    scal[SC_STATE] = ST_RUN;
+  }
+}
+
 // Jacobi preconditioner, stored as 3x3 blocks: the inverse diagonal of K on
 // the free components, zero elsewhere.  (Scalar Jacobi converged faster than
 // the inverse 3x3 diagonal blocks on the elasticity systems measured; the
@@ -425,6 +473,7 @@ struct SolveSys {
   int64_t iters = 0, updates = 0;
   int status = 0;           // 0 running, 1 converged, -1 failed
   int32_t sblk0 = 0, nsblk = 0, vblk0 = 0, nvblk = 0;
+  AmgSys* amg = nullptr;
 };
 
 BatchSolver::BatchSolver(llfem_ctx* c, bool f64_vectors) : ctx_(c), f64v_(f64_vectors) {}
@@ -433,9 +482,11 @@ BatchSolver::~BatchSolver() {
   for (SolveSys* s : sys_) release(s);
   sys_.clear();
   if (tables_) dfree(ctx_, tables_, tables_bytes_);
+  amg_batch_free(ctx_, amgb_);
 }
 
 void BatchSolver::release(SolveSys* s) {
+  amg_sys_free(ctx_, s->amg);
   if (s->mem) dfree(ctx_, s->mem, s->bytes);
   delete s;
 }
@@ -491,6 +542,17 @@ int BatchSolver::update(SolveSys* s, bool init) {
   const double pscale = s_new / s->scale;
   s->scale = s_new;
   const double tol = sp.reltol * s->bnorm * s_new;
+  if (s->amg) {
+    const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
+    double* vpart = s->part + nsblk;
+    k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, init ? 1 : 0, (float*)s->r,
+                                                      (float*)s->p, vpart);
+    LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
+    k_ru_scal_amg<<<1, kSB, 0, st>>>(n, nvblk, init ? 1 : 0, (const float*)s->z, (float*)s->p, vpart, s->scal,
+                                     tol * tol, kDelta * kDelta);
+    LLFEM_HIP(hipGetLastError());
+    return LLFEM_OK;
+  }
   const int64_t nb = blocks(sp.N);
   if (f64v_)
     k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, init ? 1 : 0, s->binv,
@@ -552,6 +614,17 @@ int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
   LLFEM_HIP(hipMemsetAsync(s->x, 0, vsz * n, st));
   LLFEM_HIP(hipMemsetAsync(s->scal, 0, sizeof(double) * SC_N, st));
   s->scale = 1.0;
+  if (spec.amg) {
+    if (f64v_) {
+      release(s);
+      return fail(LLFEM_EUNSUP, "engine: the AMG preconditioner runs with FP32 Krylov vectors");
+    }
+    rc = amg_sys_create(ctx_, spec.amg, s->val32, (float*)s->r, (float*)s->z, (float*)s->q, s->part + nsblk, &s->amg);
+    if (rc != LLFEM_OK) {
+      release(s);
+      return rc;
+    }
+  }
   rc = update(s, true);
   if (rc != LLFEM_OK) {
     release(s);
@@ -582,6 +655,7 @@ int BatchSolver::rebuild_tables() {
     d.part = s->part;
     d.nsblk = (int32_t)spmv_blocks(sp.N);
     d.nvblk = (int32_t)vec_blocks(sp.N);
+    d.amg = s->amg ? 1 : 0;
     d.sblk0 = (int32_t)sb.size();
     d.vblk0 = (int32_t)vb.size();
     const int32_t id = (int32_t)dev.size();
@@ -589,6 +663,16 @@ int BatchSolver::rebuild_tables() {
     vb.insert(vb.end(), (size_t)d.nvblk, id);
     dev.push_back(d);
     live_.push_back(s);
+  }
+  {
+    std::vector<AmgSys*> as;
+    std::vector<const double*> st_words;
+    for (SolveSys* s : live_)
+      if (s->amg) {
+        as.push_back(s->amg);
+        st_words.push_back(s->scal + SC_STATE);
+      }
+    LLFEM_TRY(amg_batch_build(ctx_, as, st_words, &amgb_));
   }
   nsys_ = (int)dev.size();
   nsb_ = (int64_t)sb.size();
@@ -637,6 +721,7 @@ int BatchSolver::run(int iters, std::vector<SolveSys*>* finished) {
       k_bs_spmv<float><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
       k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
       k_bs_update<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+      LLFEM_TRY(amg_batch_apply(ctx_, amgb_));
       k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
       k_bs_pupd<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
     }

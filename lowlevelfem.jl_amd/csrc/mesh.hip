@@ -436,6 +436,11 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_csrc, m->ncontrib);
   drelease(c, m->d_blk_node, m->nadj);
   if (m->d_ho) dfree(c, m->d_ho, m->ho_bytes);
+  for (auto& kv : m->amg) {
+    amg_free(c, kv.second.first);
+    dfree(c, kv.second.second, 3 * (size_t)m->N);
+  }
+  m->amg.clear();
 }
 
 extern "C" {

@@ -570,7 +570,7 @@ __global__ void k_i64_sub(int64_t n, int64_t* __restrict__ a, int64_t base) {
 // download u.
 int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const double* f, const int64_t* fixed,
                         int64_t nfixed, int32_t one_based, const double* u_fixed, double* u_out,
-                        llfem_stats* stats) {
+                        llfem_stats* stats, llfem_mesh* amg_mesh) {
   hipStream_t st = ctx->stream;
   const int64_t n = 3 * sp.N;
   const size_t bytes = sizeof(double) * 3 * (size_t)n + (size_t)n + sizeof(int64_t) * (size_t)(nfixed + 1) +
@@ -608,9 +608,34 @@ int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const doub
   sp.fixed = d_mask;
   sp.f = d_f;
   sp.ufix = d_ufix;
+  if (amg_mesh) {
+    // the hierarchy of this support, built on first use and kept on the mesh
+    uint64_t key = hash_bytes(&one_based, sizeof(one_based));
+    key = hash_bytes(&nfixed, sizeof(nfixed), key);
+    if (nfixed > 0) key = hash_bytes(fixed, sizeof(int64_t) * (size_t)nfixed, key);
+    auto it = amg_mesh->amg.find(key);
+    if (it == amg_mesh->amg.end()) {
+      uint8_t* keep = nullptr;
+      LLFEM_TRY(dalloc(ctx, &keep, (size_t)n));
+      LLFEM_HIP(hipMemcpyAsync(keep, d_mask, (size_t)n, hipMemcpyDeviceToDevice, st));
+      sp.fixed = keep;
+      AmgHier* h = nullptr;
+      const int rc = amg_build(ctx, sp, amg_mesh->d_xyz, &h);
+      if (rc != LLFEM_OK) {
+        (void)hipStreamSynchronize(st);
+        dfree(ctx, keep, (size_t)n);
+        return rc;
+      }
+      it = amg_mesh->amg.emplace(key, std::make_pair(h, keep)).first;
+    }
+    sp.fixed = it->second.second;
+    sp.amg = it->second.first;
+  }
   BatchSolver eng(ctx, f64_vectors);
   SolveSys* s = nullptr;
   LLFEM_TRY(eng.add(sp, &s));
+  const double setup_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   while (sys_status(s) == 0) {
     std::vector<SolveSys*> done;
     LLFEM_TRY(eng.run(32, &done));
@@ -624,6 +649,7 @@ int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const doub
     stats->refinements = sys_updates(s);
     stats->relres = sys_relres(s);
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->setup_ms = setup_ms;
   }
   const double rr = sys_relres(s);
   eng.remove(s);
@@ -655,11 +681,13 @@ int llfem_solve(llfem_mesh* m, const double* f, const int64_t* fixed, int64_t nf
   if (!m || !f || !u_out || (nfixed > 0 && !fixed)) return fail(LLFEM_EINVAL, "llfem_solve: NULL argument");
   if (!m->have_K) return fail(LLFEM_EINVAL, "llfem_solve: no stiffness assembled on this mesh");
   if (method != LLFEM_PCG_JACOBI_FP64 && method != LLFEM_PCG_MIXED_REFINE &&
-      method != LLFEM_PCG_MIXED_RELIABLE && method != LLFEM_PCG_F32OP_RELIABLE)
+      method != LLFEM_PCG_MIXED_RELIABLE && method != LLFEM_PCG_F32OP_RELIABLE &&
+      method != LLFEM_PCG_AMG_RELIABLE)
     return fail(LLFEM_EUNSUP, "llfem_solve: unknown method");
   if (!(reltol > 0)) return fail(LLFEM_EINVAL, "llfem_solve: reltol must be positive");
   LLFEM_HIP(hipSetDevice(m->ctx->device));
-  if (method == LLFEM_PCG_MIXED_RELIABLE || method == LLFEM_PCG_F32OP_RELIABLE) {
+  if (method == LLFEM_PCG_MIXED_RELIABLE || method == LLFEM_PCG_F32OP_RELIABLE ||
+      method == LLFEM_PCG_AMG_RELIABLE) {
     SysSpec sp;
     sp.N = m->N;
     sp.nadj = m->nadj;
@@ -670,7 +698,7 @@ int llfem_solve(llfem_mesh* m, const double* f, const int64_t* fixed, int64_t nf
     sp.reltol = reltol;
     sp.maxiter = maxiter;
     return solve_host_reliable(m->ctx, sp, method == LLFEM_PCG_F32OP_RELIABLE, f, fixed, nfixed, one_based,
-                               u_fixed, u_out, stats);
+                               u_fixed, u_out, stats, method == LLFEM_PCG_AMG_RELIABLE ? m : nullptr);
   }
   if (method == LLFEM_PCG_MIXED_REFINE) LLFEM_TRY(ensure_k32(m));
   PanelOp K{m->N, m->d_adj_ptr, m->d_adj, m->d_diag_slot};

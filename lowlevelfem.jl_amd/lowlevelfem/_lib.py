@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("LLFEM_LIB", os.path.join(_HERE, "libllfem.so"))
 OK, EINVAL, EHIP, ENOMEM, EGEOM, EPATTERN, ENOCONV, EUNSUP = 0, -1, -2, -3, -4, -5, -6, -7
 OP_STIFFNESS, OP_MASS = 0, 1
 FP64, FP32_INT_FP64_SUM, BF16X3 = 0, 1, 2
-PCG_JACOBI_FP64, PCG_MIXED_REFINE, PCG_MIXED_RELIABLE, PCG_F32OP_RELIABLE = 0, 1, 2, 3
+PCG_JACOBI_FP64, PCG_MIXED_REFINE, PCG_MIXED_RELIABLE, PCG_F32OP_RELIABLE, PCG_AMG_RELIABLE = 0, 1, 2, 3, 4
 
 EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
@@ -39,7 +39,7 @@ class Stats(C.Structure):
                 ("total_ms", C.c_double), ("nnz", C.c_int64), ("elements", C.c_int64),
                 ("alg_bytes", C.c_double), ("iterations", C.c_int64),
                 ("refinements", C.c_int64), ("relres", C.c_double),
-                ("alg_flops", C.c_double)]
+                ("alg_flops", C.c_double), ("setup_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
