@@ -1309,6 +1309,12 @@ bool amg_job_ready(AmgJob* j) {
   return j->fut.wait_for(std::chrono::seconds(0)) == std::future_status::ready;
 }
 
+void amg_job_drop(AmgJob* j) {
+  if (!j) return;
+  j->fut.wait();
+  delete j;
+}
+
 int amg_job_finish(llfem_ctx* ctx, AmgJob* j, const SysSpec& sp, AmgHier** out) {
   int rc = j->fut.get();
   if (rc == LLFEM_OK) rc = amg_upload(ctx, j->host, sp, out);

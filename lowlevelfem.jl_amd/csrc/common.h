@@ -266,6 +266,7 @@ struct AmgJob;
 int amg_job_start(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgJob** job);
 bool amg_job_ready(AmgJob* job);
 int amg_job_finish(llfem_ctx* ctx, AmgJob* job, const SysSpec& sp, AmgHier** out);   // frees the job
+void amg_job_drop(AmgJob* job);                                                      // waits, frees
 
 int64_t sys_iterations(const SolveSys* s);
 int64_t sys_updates(const SolveSys* s);

@@ -57,13 +57,34 @@ __global__ void k_mask_from_list(int64_t nfix, const int64_t* __restrict__ idx, 
   mask[i] = 1;
 }
 
+// AMG hierarchy of one (mesh, support): built from the stiffness of a fixed
+// material (E = 1, nu = 0.3), so it depends on the mesh and support only and
+// every request on them gets the same preconditioner.
+struct AmgEntry {
+  AmgHier* h = nullptr;
+  AmgJob* job = nullptr;    // host build in flight
+  bool failed = false;      // build failed: requests fall back to Jacobi
+};
+
 struct CachedMesh {
   llfem_mesh* m = nullptr;
   int users = 0;
   Clock::time_point last;
   std::map<uint64_t, std::pair<double*, int64_t>> weights;   // face key -> per-node load weights
   std::map<uint64_t, uint8_t*> masks;                        // support key -> constrained-DoF mask
+  std::map<uint64_t, AmgEntry> amg;                          // support key -> AMG hierarchy
 };
+
+void free_cached(llfem_ctx* ctx, CachedMesh& c) {
+  for (auto& a : c.amg) {
+    amg_job_drop(a.second.job);
+    amg_free(ctx, a.second.h);
+  }
+  c.amg.clear();
+  for (auto& w : c.weights) dfree(ctx, w.second.first, sizeof(double) * (size_t)w.second.second);
+  for (auto& mk : c.masks) dfree(ctx, mk.second, 3 * (size_t)c.m->N);
+  llfem_mesh_destroy(c.m);
+}
 
 struct Req {
   int64_t id = 0;
@@ -81,6 +102,9 @@ struct Req {
   double* d_u = nullptr;
   SolveSys* sys = nullptr;
   BatchSolver* eng = nullptr;
+  SysSpec spec;              // the system (kept while it waits for its AMG hierarchy)
+  uint64_t support_key = 0;
+  bool waiting = false;      // admitted, its AMG hierarchy still building
   int64_t iterations = 0, updates = 0;
   double relres = 0.0;
 };
@@ -152,9 +176,7 @@ bool evict_unused(llfem_server* S) {
   for (auto it = S->meshes.begin(); it != S->meshes.end();) {
     CachedMesh& c = it->second;
     if (c.users == 0) {
-      for (auto& w : c.weights) dfree(S->ctx, w.second.first, sizeof(double) * (size_t)w.second.second);
-      for (auto& mk : c.masks) dfree(S->ctx, mk.second, 3 * (size_t)c.m->N);
-      llfem_mesh_destroy(c.m);
+      free_cached(S->ctx, c);
       it = S->meshes.erase(it);
       any = true;
     } else {
@@ -264,6 +286,41 @@ int admit(llfem_server* S, Req* q) {
     sp.reltol = r.reltol;
     sp.maxiter = r.maxiter;
     q->eng = (r.method == LLFEM_PCG_F32OP_RELIABLE) ? S->eng64.get() : S->eng32.get();
+    q->support_key = sk;
+    if (r.method == LLFEM_PCG_AMG_RELIABLE) {
+      AmgEntry& ae = cm->amg[sk];
+      if (!ae.h && !ae.job && !ae.failed) {
+        double* d_Kc = nullptr;
+        if ((rc = dalloc(ctx, &d_Kc, 9 * (size_t)m->nadj))) break;
+        llfem_op_desc dc = r.stiffness;
+        dc.op = LLFEM_OP_STIFFNESS;
+        dc.D = nullptr;
+        dc.E = 1.0;
+        dc.nu = 0.3;
+        dc.precision = LLFEM_FP64;
+        int rb = assemble_impl(m, &dc, nullptr, nullptr, d_Kc);
+        if (rb == LLFEM_OK) {
+          SysSpec sc = sp;
+          sc.val64 = d_Kc;
+          rb = amg_job_start(ctx, sc, m->d_xyz, &ae.job);
+        }
+        (void)hipStreamSynchronize(st);
+        dfree(ctx, d_Kc, sizeof(double) * 9 * (size_t)m->nadj);
+        if (rb == LLFEM_ENOMEM) {
+          rc = rb;
+          break;
+        }
+        if (rb != LLFEM_OK) ae.failed = true;
+      }
+      sp.amg = ae.h;
+      if (ae.job) {
+        q->spec = sp;
+        q->waiting = true;
+        q->t_admit = Clock::now();
+        return LLFEM_OK;
+      }
+    }
+    q->spec = sp;
     if ((rc = q->eng->add(sp, &q->sys))) break;
   } while (0);
   if (rc != LLFEM_OK) {
@@ -346,13 +403,63 @@ void worker_loop(llfem_server* S) {
         fail_request(S, q.get(), rc);
         continue;
       }
-      if (sys_status(q->sys) != 0) {              // b = 0: converged at admission
+      if (q->sys && sys_status(q->sys) != 0) {    // b = 0: converged at admission
         complete(S, q.get());
         std::lock_guard<std::mutex> lk(S->mu);
         S->active.erase(std::find(S->active.begin(), S->active.end(), q));
       }
     }
+    // requests waiting for an AMG hierarchy whose host build has finished
+    bool waiting = false;
+    {
+      std::vector<std::shared_ptr<Req>> act;
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        act = S->active;
+      }
+      for (auto& q : act) {
+        if (!q->waiting) continue;
+        AmgEntry& ae = q->cm->amg[q->support_key];
+        if (ae.job && amg_job_ready(ae.job)) {
+          AmgJob* j = ae.job;
+          ae.job = nullptr;
+          if (amg_job_finish(S->ctx, j, q->spec, &ae.h) != LLFEM_OK) {
+            ae.h = nullptr;
+            ae.failed = true;
+          }
+        }
+        if (ae.job) {
+          waiting = true;
+          continue;
+        }
+        q->spec.amg = ae.h;
+        const int rc = q->eng->add(q->spec, &q->sys);
+        if (rc == LLFEM_ENOMEM && (S->eng32->any_running() || S->eng64->any_running())) {
+          waiting = true;          // retry once running systems release memory
+          continue;
+        }
+        q->waiting = false;
+        if (rc != LLFEM_OK) {
+          q->sys = nullptr;
+          free_request_buffers(S->ctx, q.get());
+          release_mesh_use(q.get());
+          {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->active.erase(std::find(S->active.begin(), S->active.end(), q));
+          }
+          fail_request(S, q.get(), rc);
+        } else if (sys_status(q->sys) != 0) {
+          complete(S, q.get());
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->active.erase(std::find(S->active.begin(), S->active.end(), q));
+        }
+      }
+    }
     if (S->active.empty()) continue;
+    if (waiting && !S->eng32->any_running() && !S->eng64->any_running()) {
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+      continue;
+    }
     // one chunk of iterations over every admitted system
     std::vector<SolveSys*> fin;
     int rc = LLFEM_OK;
@@ -363,7 +470,7 @@ void worker_loop(llfem_server* S) {
       std::lock_guard<std::mutex> lk(S->mu);
       for (auto it = S->active.begin(); it != S->active.end();) {
         Req* q = it->get();
-        const bool finished = std::find(fin.begin(), fin.end(), q->sys) != fin.end() || rc != LLFEM_OK;
+        const bool finished = q->sys && (std::find(fin.begin(), fin.end(), q->sys) != fin.end() || rc != LLFEM_OK);
         if (finished) {
           done.push_back(*it);
           it = S->active.erase(it);
@@ -395,8 +502,9 @@ int check_request(const llfem_request* r) {
     return fail(LLFEM_EINVAL, "llfem_server_submit: bad face load");
   if (r->nfixed > 0 && !r->fixed) return fail(LLFEM_EINVAL, "llfem_server_submit: NULL fixed list");
   if (!r->u_out) return fail(LLFEM_EINVAL, "llfem_server_submit: NULL u_out");
-  if (r->method != LLFEM_PCG_MIXED_RELIABLE && r->method != LLFEM_PCG_F32OP_RELIABLE)
-    return fail(LLFEM_EUNSUP, "llfem_server_submit: method must be MIXED_RELIABLE or F32OP_RELIABLE");
+  if (r->method != LLFEM_PCG_MIXED_RELIABLE && r->method != LLFEM_PCG_F32OP_RELIABLE &&
+      r->method != LLFEM_PCG_AMG_RELIABLE)
+    return fail(LLFEM_EUNSUP, "llfem_server_submit: method must be MIXED_RELIABLE, F32OP_RELIABLE or AMG_RELIABLE");
   if (!(r->reltol > 0) || r->maxiter <= 0) return fail(LLFEM_EINVAL, "llfem_server_submit: bad tolerance");
   if (!r->stiffness.weights || !r->stiffness.dN_ref || r->stiffness.q <= 0)
     return fail(LLFEM_EINVAL, "llfem_server_submit: bad quadrature tables");
@@ -480,12 +588,7 @@ extern "C" int llfem_server_destroy(llfem_server* S) {
   if (S->worker.joinable()) S->worker.join();
   S->eng32.reset();
   S->eng64.reset();
-  for (auto& kv : S->meshes) {
-    CachedMesh& c = kv.second;
-    for (auto& w : c.weights) dfree(S->ctx, w.second.first, sizeof(double) * (size_t)w.second.second);
-    for (auto& mk : c.masks) dfree(S->ctx, mk.second, 3 * (size_t)c.m->N);
-    llfem_mesh_destroy(c.m);
-  }
+  for (auto& kv : S->meshes) free_cached(S->ctx, kv.second);
   S->meshes.clear();
   llfem_ctx_destroy(S->ctx);
   delete S;

@@ -47,20 +47,22 @@ def _submit(srv, case, mat, method=_lib.PCG_MIXED_RELIABLE):
     return srv.submit(m, tab, ftab, face, [0.0, t, 0.0], fixed, E=E, nu=nu, method=method, reltol=1e-10)
 
 
-def test_alone_equals_mix_and_oracle():
+@pytest.mark.parametrize("method", [_lib.PCG_MIXED_RELIABLE, _lib.PCG_AMG_RELIABLE], ids=["jacobi", "amg"])
+def test_alone_equals_mix_and_oracle(method):
+    say(f"[serve alone/mix] method {method}")
     cases = [_case(k, c) for k, c in CASES]
     jobs = [(ci, mi) for ci in range(len(cases)) for mi in range(len(MATS))][:9]
     # alone: one server, one request at a time
     srv = _lib.Server()
     alone = {}
     for j in jobs:
-        u, st = srv.wait(_submit(srv, cases[j[0]], MATS[j[1]]))
+        u, st = srv.wait(_submit(srv, cases[j[0]], MATS[j[1]], method))
         assert st.relres < 1e-10 and st.status == 0
         alone[j] = u
     srv.close()
-    # mix: every request in flight at once on a fresh server
+    # mix: every request in flight at once on a fresh server, in reverse order
     srv = _lib.Server()
-    ids = {j: _submit(srv, cases[j[0]], MATS[j[1]]) for j in jobs}
+    ids = {j: _submit(srv, cases[j[0]], MATS[j[1]], method) for j in reversed(jobs)}
     hits = 0
     for j, rid in ids.items():
         u, st = srv.wait(rid)
@@ -127,14 +129,16 @@ def test_bad_request_fails_alone():
 
 def test_arrivals_while_running():
     """Requests submitted from another thread while earlier ones iterate are
-    admitted into the running batch and complete correctly."""
+    admitted into the running batch and complete correctly (Jacobi and AMG
+    requests mixed in one batch)."""
     cases = [_case(k, c) for k, c in CASES]
     srv = _lib.Server()
     ids, lock = [], threading.Lock()
 
     def producer():
         for i in range(6):
-            rid = _submit(srv, cases[i % 3], MATS[i % 4])
+            method = _lib.PCG_AMG_RELIABLE if i % 2 else _lib.PCG_MIXED_RELIABLE
+            rid = _submit(srv, cases[i % 3], MATS[i % 4], method)
             with lock:
                 ids.append(rid)
             time.sleep(0.002)
