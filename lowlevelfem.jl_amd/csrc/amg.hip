@@ -33,6 +33,7 @@
 #include <cstring>
 #include <future>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -410,54 +411,105 @@ struct HostLevel {
   std::vector<int32_t> gblk, gcol;
 };
 
-// Greedy aggregation over real nodes.  strong / full: CSR graphs without self
-// loops.  Returns the aggregate count; agg = -1 for nodes with no neighbour.
-int64_t aggregate(int64_t n, const std::vector<int64_t>& sp, const std::vector<int32_t>& si,
-                  const std::vector<int64_t>& fp, const std::vector<int32_t>& fi, std::vector<int32_t>& agg) {
+// A graph over real nodes: the neighbours of i are idx[e], e in
+// [ptr[i], ptr[i+1]), except i itself and, with `flag`, entries flagged 0.
+struct Graph {
+  const int64_t* ptr = nullptr;
+  const int32_t* idx = nullptr;
+  const uint8_t* flag = nullptr;
+  template <class F>
+  void each(int64_t i, F&& f) const {
+    for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+      const int32_t j = idx[e];
+      if (j == i || (flag && !flag[e])) continue;
+      f(j);
+    }
+  }
+};
+
+// The aggregate the neighbours of i (through `from`) belong to most often;
+// ties to the smallest id; -1 if none.
+int32_t most_coupled(const Graph& g, int64_t i, const std::vector<int32_t>& from, std::vector<int32_t>& buf) {
+  buf.clear();
+  g.each(i, [&](int32_t j) {
+    if (from[j] >= 0) buf.push_back(from[j]);
+  });
+  if (buf.empty()) return -1;
+  std::sort(buf.begin(), buf.end());
+  int32_t best = -1;
+  size_t bc = 0;
+  for (size_t k = 0; k < buf.size();) {
+    size_t e = k;
+    while (e < buf.size() && buf[e] == buf[k]) ++e;
+    if (e - k > bc) {
+      bc = e - k;
+      best = buf[k];
+    }
+    k = e;
+  }
+  return best;
+}
+
+// Greedy aggregation over real nodes.  Returns the aggregate count; agg = -1
+// for nodes with no neighbour at all.
+int64_t aggregate(int64_t n, const Graph& S, const Graph& F, std::vector<int32_t>& agg) {
   agg.assign((size_t)n, -1);
   int64_t na = 0;
   for (int64_t i = 0; i < n; ++i) {   // phase 1: root + all strong neighbours, all unaggregated
-    if (agg[i] >= 0 || sp[i + 1] == sp[i]) continue;
-    bool ok = true;
-    for (int64_t e = sp[i]; e < sp[i + 1] && ok; ++e) ok = agg[si[e]] < 0;
-    if (!ok) continue;
+    if (agg[i] >= 0) continue;
+    bool ok = true, any = false;
+    S.each(i, [&](int32_t j) {
+      any = true;
+      ok = ok && agg[j] < 0;
+    });
+    if (!ok || !any) continue;
     agg[i] = (int32_t)na;
-    for (int64_t e = sp[i]; e < sp[i + 1]; ++e) agg[si[e]] = (int32_t)na;
+    S.each(i, [&](int32_t j) { agg[j] = (int32_t)na; });
     ++na;
   }
-  std::vector<int32_t> snap = agg;
-  std::vector<int32_t> cnt;
-  auto best_of = [&](int64_t i, const std::vector<int64_t>& p, const std::vector<int32_t>& ix,
-                     const std::vector<int32_t>& from) -> int32_t {
-    int32_t best = -1, bc = 0;
-    for (int64_t e = p[i]; e < p[i + 1]; ++e) {
-      const int32_t a = from[ix[e]];
-      if (a < 0) continue;
-      int32_t c = 0;
-      for (int64_t f = p[i]; f < p[i + 1]; ++f) c += from[ix[f]] == a;
-      if (c > bc || (c == bc && a < best)) { bc = c; best = a; }
-    }
-    return best;
-  };
+  std::vector<int32_t> snap = agg, buf;
   for (int64_t i = 0; i < n; ++i)     // phase 2: join the phase-1 aggregate coupled most strongly
-    if (agg[i] < 0) agg[i] = best_of(i, sp, si, snap);
+    if (agg[i] < 0) agg[i] = most_coupled(S, i, snap, buf);
   for (int64_t i = 0; i < n; ++i) {   // phase 3a: leftovers with unaggregated strong neighbours
     if (agg[i] >= 0) continue;
     bool any = false;
-    for (int64_t e = sp[i]; e < sp[i + 1]; ++e) any |= agg[si[e]] < 0;
+    S.each(i, [&](int32_t j) { any = any || agg[j] < 0; });
     if (!any) continue;
     agg[i] = (int32_t)na;
-    for (int64_t e = sp[i]; e < sp[i + 1]; ++e)
-      if (agg[si[e]] < 0) agg[si[e]] = (int32_t)na;
+    S.each(i, [&](int32_t j) {
+      if (agg[j] < 0) agg[j] = (int32_t)na;
+    });
     ++na;
   }
   for (int64_t i = 0; i < n; ++i) {   // phase 3b: weakly coupled nodes join, isolated ones stay out
     if (agg[i] >= 0) continue;
-    const int32_t a = best_of(i, fp, fi, agg);
-    if (a >= 0) agg[i] = a;
-    else if (fp[i + 1] > fp[i]) agg[i] = (int32_t)na++;
+    const int32_t a = most_coupled(F, i, agg, buf);
+    if (a >= 0) {
+      agg[i] = a;
+    } else {
+      bool any = false;
+      F.each(i, [&](int32_t) { any = true; });
+      if (any) agg[i] = (int32_t)na++;
+    }
   }
   return na;
+}
+
+// Run f(lo, hi) over [0, n) split across host threads.
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), 16));
+  if (n < 4096 || nt == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t step = (n + nt - 1) / nt;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t lo = t * step, hi = std::min(n, lo + step);
+    if (lo < hi) th.emplace_back([&f, lo, hi]() { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
 }
 
 // Modified Gram-Schmidt (two passes) of a rows x 6 matrix (row-major) in
@@ -537,16 +589,17 @@ struct AmgHier {
 
 namespace {
 
-// Build level l+1 from level l.  `real` maps the level's real nodes to
-// pseudo-nodes (level 0: one each; coarser: two each); Bn: near-null space
-// rows of the real nodes ([n_real][b][6]); graphs over real nodes.
-void coarsen(HostLevel& L, const std::vector<int64_t>& adj_ptr, const std::vector<int32_t>& adj, int b,
-             int64_t n_real, const std::vector<double>& Bn, const std::vector<int64_t>& sp,
-             const std::vector<int32_t>& si, const std::vector<int64_t>& fp, const std::vector<int32_t>& fi,
-             HostLevel& C, std::vector<double>& Bc, std::vector<int64_t>& cfp, std::vector<int32_t>& cfi) {
-  const int ps = b / 3;   // pseudo-nodes per real node
+// Build level l+1 from level l.  Real nodes map to `ps` pseudo-nodes each
+// (level 0: one; coarser: two).  nullrow(j, c, out6): near-null-space row of
+// pseudo-node j, component c.  S, F: strong and full real-node graphs.
+// Outputs the coarse level C, its near-null space Bc ([na][6][6]) and the
+// aggregate graph (nbp, nbi; each list includes the aggregate itself).
+template <class NullRow>
+void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, int64_t n_real, NullRow&& nullrow,
+             const Graph& S, const Graph& F, HostLevel& C, std::vector<double>& Bc, std::vector<int64_t>& nbp,
+             std::vector<int32_t>& nbi) {
   std::vector<int32_t> ragg;
-  const int64_t na = aggregate(n_real, sp, si, fp, fi, ragg);
+  const int64_t na = aggregate(n_real, S, F, ragg);
   L.nagg = na;
   L.agg.assign((size_t)L.N, -1);
   for (int64_t a = 0; a < n_real; ++a)
@@ -562,113 +615,108 @@ void coarsen(HostLevel& L, const std::vector<int64_t>& adj_ptr, const std::vecto
     for (int64_t j = 0; j < L.N; ++j)
       if (L.agg[j] >= 0) L.mem[fill[L.agg[j]]++] = (int32_t)j;
   }
-  // tentative prolongator and coarse near-null space
+  // tentative prolongator and coarse near-null space (aggregates in parallel)
   L.Q.assign((size_t)L.N * 18, 0.f);
   Bc.assign((size_t)na * 36, 0.0);
   C.N = 2 * na;
   C.mask.assign((size_t)C.N * 3, 0);
-  std::vector<double> M;
-  for (int64_t a = 0; a < na; ++a) {
-    const int64_t m0 = L.mem_ptr[a], m1 = L.mem_ptr[a + 1];
-    const int64_t rows = 3 * (m1 - m0);
-    M.assign((size_t)rows * 6, 0.0);
-    for (int64_t m = m0; m < m1; ++m) {
-      const int64_t j = L.mem[m];
-      const int64_t real = j / ps, sub = j % ps;
-      for (int c = 0; c < 3; ++c)
-        for (int u = 0; u < 6; ++u) M[6 * (3 * (m - m0) + c) + u] = Bn[((size_t)real * b + 3 * sub + c) * 6 + u];
+  parallel_for(na, [&](int64_t lo, int64_t hi) {
+    std::vector<double> M;
+    for (int64_t a = lo; a < hi; ++a) {
+      const int64_t m0 = L.mem_ptr[a], m1 = L.mem_ptr[a + 1];
+      const int64_t rows = 3 * (m1 - m0);
+      M.assign((size_t)rows * 6, 0.0);
+      for (int64_t m = m0; m < m1; ++m)
+        for (int c = 0; c < 3; ++c) nullrow(L.mem[m], c, &M[6 * (3 * (m - m0) + c)]);
+      double R[36];
+      bool dropped[6];
+      mgs6(rows, M.data(), R, dropped);
+      for (int64_t m = m0; m < m1; ++m) {
+        const int64_t j = L.mem[m];
+        for (int c = 0; c < 3; ++c)
+          for (int u = 0; u < 6; ++u) L.Q[18 * j + 6 * c + u] = (float)M[6 * (3 * (m - m0) + c) + u];
+      }
+      for (int k = 0; k < 36; ++k) Bc[36 * a + k] = R[k];
+      for (int u = 0; u < 6; ++u) C.mask[6 * a + u] = dropped[u] ? 1 : 0;
     }
-    double R[36];
-    bool dropped[6];
-    mgs6(rows, M.data(), R, dropped);
-    for (int64_t m = m0; m < m1; ++m) {
-      const int64_t j = L.mem[m];
-      for (int c = 0; c < 3; ++c)
-        for (int u = 0; u < 6; ++u) L.Q[18 * j + 6 * c + u] = (float)M[6 * (3 * (m - m0) + c) + u];
-    }
-    for (int k = 0; k < 36; ++k) Bc[36 * a + k] = R[k];
-    for (int u = 0; u < 6; ++u) C.mask[6 * a + u] = dropped[u] ? 1 : 0;
-  }
-  // coarse aggregate neighbour lists (through every fine block)
-  std::vector<std::vector<int32_t>> nb((size_t)na);
+  });
+  // per aggregate: neighbour aggregates (every fine block couples), then the
+  // Galerkin contribution lists of its pairs, fine blocks in ascending order
+  nbp.assign((size_t)na + 1, 0);
+  nbi.clear();
+  nbi.reserve((size_t)na * 32);
+  L.gptr.assign(1, 0);
+  L.gblk.clear();
+  L.gcol.clear();
+  L.gblk.reserve((size_t)adj_ptr[L.N]);
+  L.gcol.reserve((size_t)adj_ptr[L.N]);
+  std::vector<int32_t> stamp((size_t)na, -1), pos((size_t)na, 0);
+  std::vector<int64_t> cur;
   for (int64_t a = 0; a < na; ++a) {
-    auto& v = nb[a];
+    const size_t b0 = nbi.size();
+    stamp[a] = (int32_t)a;
+    nbi.push_back((int32_t)a);
     for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
       const int64_t j = L.mem[m];
       for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
         const int32_t B = L.agg[adj[g]];
-        if (B >= 0) v.push_back(B);
+        if (B >= 0 && stamp[B] != a) {
+          stamp[B] = (int32_t)a;
+          nbi.push_back(B);
+        }
       }
     }
-    v.push_back((int32_t)a);
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
+    std::sort(nbi.begin() + b0, nbi.end());
+    const size_t k1 = nbi.size() - b0;
+    nbp[a + 1] = (int64_t)nbi.size();
+    for (size_t k = 0; k < k1; ++k) pos[nbi[b0 + k]] = (int32_t)k;
+    cur.assign(k1 + 1, 0);
+    for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
+      const int64_t j = L.mem[m];
+      for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
+        const int32_t B = L.agg[adj[g]];
+        if (B >= 0) cur[pos[B] + 1]++;
+      }
+    }
+    const int64_t base = L.gptr.back();
+    for (size_t k = 0; k < k1; ++k) {
+      cur[k + 1] += cur[k];
+      L.gptr.push_back(base + cur[k + 1]);
+    }
+    L.gblk.resize((size_t)(base + cur[k1]));
+    L.gcol.resize((size_t)(base + cur[k1]));
+    for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
+      const int64_t j = L.mem[m];
+      for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
+        const int32_t B = L.agg[adj[g]];
+        if (B < 0) continue;
+        const int64_t o = base + cur[pos[B]]++;
+        L.gblk[o] = (int32_t)g;
+        L.gcol[o] = (int32_t)j;
+      }
+    }
   }
   // coarse pseudo-node pattern: 2A and 2A+1 both list [2B, 2B+1 for B in nb(A)]
+  const int64_t npairs = nbp[na];
   C.adj_ptr.assign((size_t)C.N + 1, 0);
-  for (int64_t a = 0; a < na; ++a) {
-    const int64_t r = 2 * (int64_t)nb[a].size();
-    C.adj_ptr[2 * a + 1] = C.adj_ptr[2 * a] + r;
-    C.adj_ptr[2 * a + 2] = C.adj_ptr[2 * a + 1] + r;
-  }
-  C.adj.assign((size_t)C.adj_ptr[C.N], 0);
+  C.adj.assign((size_t)4 * npairs, 0);
   C.dslot.assign((size_t)C.N, 0);
-  std::vector<int64_t> pbase((size_t)na + 1, 0);
+  L.pair_o.assign((size_t)npairs * 2, 0);
   for (int64_t a = 0; a < na; ++a) {
-    pbase[a + 1] = pbase[a] + (int64_t)nb[a].size();
+    const int64_t r = 2 * (nbp[a + 1] - nbp[a]);
+    C.adj_ptr[2 * a] = 4 * nbp[a];
+    C.adj_ptr[2 * a + 1] = 4 * nbp[a] + r;
+    C.adj_ptr[2 * a + 2] = 4 * nbp[a + 1];
     for (int s = 0; s < 2; ++s) {
       const int64_t o = C.adj_ptr[2 * a + s];
-      for (size_t k = 0; k < nb[a].size(); ++k) {
-        C.adj[o + 2 * k] = 2 * nb[a][k];
-        C.adj[o + 2 * k + 1] = 2 * nb[a][k] + 1;
-        if (nb[a][k] == a) C.dslot[2 * a + s] = (int32_t)(2 * k + s);
+      for (int64_t k = 0; k < nbp[a + 1] - nbp[a]; ++k) {
+        const int32_t B = nbi[nbp[a] + k];
+        C.adj[o + 2 * k] = 2 * B;
+        C.adj[o + 2 * k + 1] = 2 * B + 1;
+        if (B == a) C.dslot[2 * a + s] = (int32_t)(2 * k + s);
+        L.pair_o[2 * (nbp[a] + k) + s] = (int32_t)(o + 2 * k);
       }
     }
-  }
-  const int64_t npairs = pbase[na];
-  L.pair_o.assign((size_t)npairs * 2, 0);
-  for (int64_t a = 0; a < na; ++a)
-    for (size_t k = 0; k < nb[a].size(); ++k) {
-      L.pair_o[2 * (pbase[a] + k)] = (int32_t)(C.adj_ptr[2 * a] + 2 * k);
-      L.pair_o[2 * (pbase[a] + k) + 1] = (int32_t)(C.adj_ptr[2 * a + 1] + 2 * k);
-    }
-  // Galerkin contribution lists, fine blocks in ascending order per pair
-  const int64_t nadj = adj_ptr[L.N];
-  std::vector<int32_t> pof((size_t)nadj, -1);
-  L.gptr.assign((size_t)npairs + 1, 0);
-  for (int64_t j = 0; j < L.N; ++j) {
-    const int32_t A = L.agg[j];
-    if (A < 0) continue;
-    const auto& v = nb[A];
-    for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
-      const int32_t B = L.agg[adj[g]];
-      if (B < 0) continue;
-      const int64_t k = std::lower_bound(v.begin(), v.end(), B) - v.begin();
-      pof[g] = (int32_t)(pbase[A] + k);
-      L.gptr[pof[g] + 1]++;
-    }
-  }
-  for (int64_t p = 0; p < npairs; ++p) L.gptr[p + 1] += L.gptr[p];
-  L.gblk.assign((size_t)L.gptr[npairs], 0);
-  L.gcol.assign((size_t)L.gptr[npairs], 0);
-  {
-    std::vector<int64_t> fill(L.gptr.begin(), L.gptr.end() - 1);
-    for (int64_t j = 0; j < L.N; ++j)
-      for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
-        const int32_t p = pof[g];
-        if (p < 0) continue;
-        L.gblk[fill[p]] = (int32_t)g;
-        L.gcol[fill[p]] = (int32_t)j;
-        ++fill[p];
-      }
-  }
-  // next real-node graph: the aggregate graph without self loops
-  cfp.assign((size_t)na + 1, 0);
-  cfi.clear();
-  for (int64_t a = 0; a < na; ++a) {
-    for (int32_t B : nb[a])
-      if (B != a) cfi.push_back(B);
-    cfp[a + 1] = (int64_t)cfi.size();
   }
 }
 
@@ -708,52 +756,44 @@ int amg_build_host(const AmgInput& in, AmgHost* H) {
   const int64_t N = in.N;
   H->N0 = N;
   H->lev.clear();
+  H->lev.reserve(kMaxLevels + 1);
   H->lev.emplace_back();
   H->lev[0].N = N;
-  H->lev[0].dslot = in.dslot;
-  // graphs over mesh nodes: strong and full, without self loops
-  std::vector<int64_t> sp((size_t)N + 1, 0), fp((size_t)N + 1, 0);
-  std::vector<int32_t> si, fi;
-  si.reserve(in.adj.size());
-  fi.reserve(in.adj.size());
-  for (int64_t j = 0; j < N; ++j) {
-    for (int64_t g = in.adj_ptr[j]; g < in.adj_ptr[j + 1]; ++g) {
-      if (in.adj[g] == j) continue;
-      fi.push_back(in.adj[g]);
-      if (in.strong[g]) si.push_back(in.adj[g]);
-    }
-    sp[j + 1] = (int64_t)si.size();
-    fp[j + 1] = (int64_t)fi.size();
-  }
   // rigid-body modes about the centroid, constrained rows zeroed
-  double c[3] = {0, 0, 0};
+  double c0[3] = {0, 0, 0};
   for (int k = 0; k < 3; ++k) {
-    for (int64_t j = 0; j < N; ++j) c[k] += in.xyz[(size_t)k * N + j];
-    c[k] /= std::max<int64_t>(N, 1);
+    for (int64_t j = 0; j < N; ++j) c0[k] += in.xyz[(size_t)k * N + j];
+    c0[k] /= std::max<int64_t>(N, 1);
   }
-  std::vector<double> Bn((size_t)N * 18, 0.0);
-  for (int64_t j = 0; j < N; ++j) {
-    const double r[3] = {in.xyz[j] - c[0], in.xyz[(size_t)N + j] - c[1], in.xyz[(size_t)2 * N + j] - c[2]};
-    double* B = &Bn[(size_t)j * 18];   // [d][6]
-    for (int d = 0; d < 3; ++d) B[6 * d + d] = 1.0;
-    B[6 * 1 + 3] = -r[2]; B[6 * 2 + 3] = r[1];
-    B[6 * 0 + 4] = r[2];  B[6 * 2 + 4] = -r[0];
-    B[6 * 0 + 5] = -r[1]; B[6 * 1 + 5] = r[0];
-    for (int d = 0; d < 3; ++d)
-      if (in.mask[3 * j + d])
-        for (int u = 0; u < 6; ++u) B[6 * d + u] = 0.0;
-  }
-  int b = 3;
-  int64_t n_real = N;
-  const std::vector<int64_t>* adj_ptr = &in.adj_ptr;
-  const std::vector<int32_t>* adj = &in.adj;
+  auto rbm_row = [&](int64_t j, int c, double* out) {
+    for (int u = 0; u < 6; ++u) out[u] = 0.0;
+    if (in.mask[3 * j + c]) return;
+    const double r[3] = {in.xyz[j] - c0[0], in.xyz[(size_t)N + j] - c0[1], in.xyz[(size_t)2 * N + j] - c0[2]};
+    out[c] = 1.0;
+    if (c == 0) { out[4] = r[2]; out[5] = -r[1]; }
+    if (c == 1) { out[3] = -r[2]; out[5] = r[0]; }
+    if (c == 2) { out[3] = r[1]; out[4] = -r[0]; }
+  };
+  std::vector<double> Bn, Bc;
+  std::vector<int64_t> nbp, nbp_prev;
+  std::vector<int32_t> nbi, nbi_prev;
   for (int l = 0;; ++l) {
     HostLevel C;
-    std::vector<double> Bc;
-    std::vector<int64_t> cfp;
-    std::vector<int32_t> cfi;
-    coarsen(H->lev[l], *adj_ptr, *adj, b, n_real, Bn, sp, si, fp, fi, C, Bc, cfp, cfi);
+    if (l == 0) {
+      const Graph S{in.adj_ptr.data(), in.adj.data(), in.strong.data()};
+      const Graph F{in.adj_ptr.data(), in.adj.data(), nullptr};
+      coarsen(H->lev[0], in.adj_ptr.data(), in.adj.data(), 1, N, rbm_row, S, F, C, Bc, nbp, nbi);
+    } else {
+      const HostLevel& L = H->lev[l];
+      const Graph G{nbp_prev.data(), nbi_prev.data(), nullptr};
+      auto row = [&](int64_t j, int c, double* out) {   // real node j/2, DoF 3 (j%2) + c
+        const double* R = &Bn[(size_t)(j / 2) * 36 + 6 * (3 * (j % 2) + c)];
+        for (int u = 0; u < 6; ++u) out[u] = R[u];
+      };
+      coarsen(H->lev[l], L.adj_ptr.data(), L.adj.data(), 2, L.N / 2, row, G, G, C, Bc, nbp, nbi);
+    }
     const int64_t na = H->lev[l].nagg;
+    const int64_t n_real = l == 0 ? N : H->lev[l].N / 2;
     if (na == 0) return fail(LLFEM_EUNSUP, "AMG: no aggregates (empty operator)");
     H->lev.push_back(std::move(C));
     if (na <= kCoarseAggs) break;
@@ -761,16 +801,9 @@ int amg_build_host(const AmgInput& in, AmgHost* H) {
       if (6 * na > kMaxDense) return fail(LLFEM_EUNSUP, "AMG: coarsening stalled");
       break;
     }
-    // the next level aggregates the aggregate graph (all couplings strong)
     Bn.swap(Bc);
-    b = 6;
-    n_real = na;
-    sp = cfp;
-    si = cfi;
-    fp.swap(cfp);
-    fi.swap(cfi);
-    adj_ptr = &H->lev.back().adj_ptr;
-    adj = &H->lev.back().adj;
+    nbp_prev.swap(nbp);
+    nbi_prev.swap(nbi);
   }
   // every index the kernels follow stays inside its array
   for (size_t l = 0; l + 1 < H->lev.size(); ++l) {

@@ -208,7 +208,8 @@ struct SolveSys;
 
 class BatchSolver {
  public:
-  static constexpr double kDelta = 0.1;   // reliable-update trigger (residual reduction)
+  static constexpr double kDelta = 0.1;      // reliable-update trigger (residual reduction)
+  static constexpr int kMaxRunAmg = 40;      // AMG systems: FP64 update after at most this many iterations
   // f64_vectors: FP64 Krylov vectors over the FP32 operator (else FP32 vectors)
   BatchSolver(llfem_ctx* c, bool f64_vectors);
   ~BatchSolver();

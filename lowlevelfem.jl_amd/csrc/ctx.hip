@@ -33,13 +33,29 @@ void dfree(llfem_ctx* ctx, void* p, size_t bytes) {
   ctx->in_use -= bytes;
 }
 
+// Content hash for cache keys (not cryptographic): four independent 64-bit
+// multiply-rotate lanes over 32-byte strides, byte-wise FNV-1a on the tail,
+// then a final avalanche.  Several GB/s, so hashing a request's mesh arrays
+// costs milliseconds.
 uint64_t hash_bytes(const void* p, size_t n, uint64_t h) {
+  constexpr uint64_t kA = 0x9E3779B97F4A7C15ull, kB = 0xBF58476D1CE4E5B9ull, kC = 0x94D049BB133111EBull;
+  auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };
   const unsigned char* c = static_cast<const unsigned char*>(p);
-  for (size_t i = 0; i < n; ++i) {
-    h ^= c[i];
-    h *= 1099511628211ull;
-  }
-  return h;
+  uint64_t lane[4] = {h ^ kA, h + kB, rotl(h, 21) ^ kC, h - kA};
+  size_t i = 0;
+  for (; i + 32 <= n; i += 32)
+    for (int k = 0; k < 4; ++k) {
+      uint64_t w;
+      std::memcpy(&w, c + i + 8 * k, 8);
+      lane[k] = rotl(lane[k] + w * kB, 31) * kA;
+    }
+  uint64_t r = h ^ ((uint64_t)n * kC);
+  for (int k = 0; k < 4; ++k) r = (r ^ rotl(lane[k], 17 + 8 * k)) * kB;
+  for (; i < n; ++i) r = (r ^ c[i]) * 1099511628211ull;
+  r ^= r >> 31;
+  r *= kC;
+  r ^= r >> 29;
+  return r;
 }
 
 }  // namespace llfem

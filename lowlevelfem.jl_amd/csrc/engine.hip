@@ -47,7 +47,10 @@ constexpr int kVecNodes = kSB;           // nodes per vector block pass
 constexpr int64_t kMaxSpmvBlocks = 2048;
 constexpr int64_t kMaxVecBlocks = 1024;
 
-enum { SC_RHO = 0, SC_ALPHA, SC_BETA, SC_RR, SC_MAXRR, SC_TOL2, SC_DELTA2, SC_STATE, SC_ITER, SC_PQ, SC_N = 16 };
+// scal words of a system; SC_LAST: iteration count at the last FP64 update,
+// SC_MAXRUN: force an update after this many iterations (0: never)
+enum { SC_RHO = 0, SC_ALPHA, SC_BETA, SC_RR, SC_MAXRR, SC_TOL2, SC_DELTA2, SC_STATE, SC_ITER, SC_PQ, SC_LAST,
+       SC_MAXRUN, SC_N = 16 };
 enum { ST_RUN = 0, ST_PENDING = 1, ST_PAUSED = 2, ST_IDLE = 3 };
 
 struct SysDev {
@@ -210,7 +213,9 @@ __global__ __launch_bounds__(kSB) void k_bs_beta(const SysDev* __restrict__ sys)
     S.scal[SC_RHO] = rz;
     S.scal[SC_RR] = rr;
     S.scal[SC_ITER] += 1.0;
-    if (rr <= S.scal[SC_TOL2] || rr < S.scal[SC_DELTA2] * S.scal[SC_MAXRR] || !(rz > 0.0))
+    const double maxrun = S.scal[SC_MAXRUN];
+    if (rr <= S.scal[SC_TOL2] || rr < S.scal[SC_DELTA2] * S.scal[SC_MAXRR] || !(rz > 0.0) ||
+        (maxrun > 0.0 && S.scal[SC_ITER] - S.scal[SC_LAST] >= maxrun))
       S.scal[SC_STATE] = ST_PENDING;
   }
 }
@@ -382,7 +387,8 @@ __global__ __launch_bounds__(kSB) void k_ru_restart_amg(int64_t N, int32_t nvblk
 // partials (one block).
 __global__ __launch_bounds__(kSB) void k_ru_scal_amg(int64_t n, int32_t nvblk, int init, const float* __restrict__ z,
                                                      float* __restrict__ p, const double* __restrict__ part,
-                                                     double* __restrict__ scal, double tol2, double delta2) {
+                                                     double* __restrict__ scal, double tol2, double delta2,
+                                                     double maxrun) {
   __shared__ double sh[kSB];
   if (init)
     for (int64_t i = threadIdx.x; i < n; i += kSB) p[i] = z[i];
@@ -400,6 +406,8 @@ __global__ __launch_bounds__(kSB) void k_ru_scal_amg(int64_t n, int32_t nvblk, i
     scal[SC_TOL2] = tol2;
     scal[SC_DELTA2] = delta2;
     scal[SC_BETA] = 0.0;
+    scal[SC_LAST] = scal[SC_ITER];
+    scal[SC_MAXRUN] = maxrun;
     scal[SC_STATE] = ST_RUN;
   }
 }
@@ -549,7 +557,7 @@ int BatchSolver::update(SolveSys* s, bool init) {
                                                       (float*)s->p, vpart);
     LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
     k_ru_scal_amg<<<1, kSB, 0, st>>>(n, nvblk, init ? 1 : 0, (const float*)s->z, (float*)s->p, vpart, s->scal,
-                                     tol * tol, kDelta * kDelta);
+                                     tol * tol, kDelta * kDelta, (double)kMaxRunAmg);
     LLFEM_HIP(hipGetLastError());
     return LLFEM_OK;
   }

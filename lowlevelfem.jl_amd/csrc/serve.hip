@@ -21,6 +21,8 @@
 // first).  A system's arithmetic does not depend on the other systems in the
 // batch, so a request's result is the same alone and in any mix.
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -37,6 +39,17 @@ namespace llfem {
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// LLFEM_SERVE_TRACE=1: per-request admission phases and per-chunk times on stderr.
+bool trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("LLFEM_SERVE_TRACE");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
 
 __global__ void k_load_from_weights(int64_t N, const double* __restrict__ s, double tx, double ty, double tz,
                                     double* __restrict__ f) {
@@ -116,7 +129,7 @@ struct Req {
 struct llfem_server {
   llfem_ctx* ctx = nullptr;
   int32_t max_active = 64;
-  int32_t chunk = 16;
+  int32_t chunk = 8;         // iterations between host checks (AMG systems update every ~10)
   std::unique_ptr<llfem::BatchSolver> eng32, eng64;   // FP32 / FP64 Krylov vectors
   std::mutex mu;
   std::condition_variable cv;
@@ -192,6 +205,8 @@ int admit(llfem_server* S, Req* q) {
   llfem_ctx* ctx = S->ctx;
   hipStream_t st = ctx->stream;
   const llfem_request& r = q->in;
+  const Clock::time_point t0 = Clock::now();
+  double t_mesh = 0, t_wm = 0, t_k = 0, t_amg = 0;
   q->mesh_key = mesh_hash(r);
   auto it = S->meshes.find(q->mesh_key);
   if (it == S->meshes.end()) {
@@ -205,6 +220,7 @@ int admit(llfem_server* S, Req* q) {
     q->cache_hit = true;
   }
   CachedMesh* cm = &it->second;
+  t_mesh = ms_since(t0);
   q->cm = cm;
   cm->users++;
   cm->last = Clock::now();
@@ -264,6 +280,7 @@ int admit(llfem_server* S, Req* q) {
       }
       mi = cm->masks.emplace(sk, d_mask).first;
     }
+    t_wm = ms_since(t0);
     // K for this request's material, f = t (x) weights, u
     if ((rc = dalloc(ctx, &q->d_K, 9 * (size_t)m->nadj))) break;
     if ((rc = dalloc(ctx, &q->d_f, (size_t)n))) break;
@@ -287,6 +304,7 @@ int admit(llfem_server* S, Req* q) {
     sp.maxiter = r.maxiter;
     q->eng = (r.method == LLFEM_PCG_F32OP_RELIABLE) ? S->eng64.get() : S->eng32.get();
     q->support_key = sk;
+    t_k = ms_since(t0);
     if (r.method == LLFEM_PCG_AMG_RELIABLE) {
       AmgEntry& ae = cm->amg[sk];
       if (!ae.h && !ae.job && !ae.failed) {
@@ -313,15 +331,23 @@ int admit(llfem_server* S, Req* q) {
         if (rb != LLFEM_OK) ae.failed = true;
       }
       sp.amg = ae.h;
+      t_amg = ms_since(t0);
       if (ae.job) {
         q->spec = sp;
         q->waiting = true;
         q->t_admit = Clock::now();
+        if (trace_on())
+          std::fprintf(stderr, "[serve] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f -> waits\n",
+                       (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg - t_k);
         return LLFEM_OK;
       }
     }
     q->spec = sp;
     if ((rc = q->eng->add(sp, &q->sys))) break;
+    if (trace_on())
+      std::fprintf(stderr, "[serve] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f add %.1f ms\n",
+                   (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg > 0 ? t_amg - t_k : 0.0,
+                   ms_since(t0) - (t_amg > 0 ? t_amg : t_k));
   } while (0);
   if (rc != LLFEM_OK) {
     (void)hipStreamSynchronize(st);
@@ -433,7 +459,10 @@ void worker_loop(llfem_server* S) {
           continue;
         }
         q->spec.amg = ae.h;
+        const Clock::time_point ta = Clock::now();
         const int rc = q->eng->add(q->spec, &q->sys);
+        if (trace_on())
+          std::fprintf(stderr, "[serve] start #%lld after hierarchy: add %.1f ms\n", (long long)q->id, ms_since(ta));
         if (rc == LLFEM_ENOMEM && (S->eng32->any_running() || S->eng64->any_running())) {
           waiting = true;          // retry once running systems release memory
           continue;
@@ -463,8 +492,12 @@ void worker_loop(llfem_server* S) {
     // one chunk of iterations over every admitted system
     std::vector<SolveSys*> fin;
     int rc = LLFEM_OK;
+    const Clock::time_point tc = Clock::now();
     for (BatchSolver* e : {S->eng32.get(), S->eng64.get()})
       if (rc == LLFEM_OK && e->any_running()) rc = e->run(S->chunk, &fin);
+    if (trace_on())
+      std::fprintf(stderr, "[serve] chunk %.1f ms, %zu active, %zu finished\n", ms_since(tc), S->active.size(),
+                   fin.size());
     std::vector<std::shared_ptr<Req>> done;
     {
       std::lock_guard<std::mutex> lk(S->mu);

@@ -636,9 +636,12 @@ int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const doub
   LLFEM_TRY(eng.add(sp, &s));
   const double setup_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // AMG systems reach the reliable-update trigger every ~10 iterations: short
+  // chunks keep a paused system from idling through the rest of a chunk
+  const int chunk = sp.amg ? 8 : 32;
   while (sys_status(s) == 0) {
     std::vector<SolveSys*> done;
-    LLFEM_TRY(eng.run(32, &done));
+    LLFEM_TRY(eng.run(chunk, &done));
   }
   LLFEM_TRY(eng.finish(s, d_u));
   LLFEM_HIP(hipMemcpyAsync(u_out, d_u, sizeof(double) * n, hipMemcpyDeviceToHost, st));
