@@ -177,25 +177,35 @@ __global__ __launch_bounds__(kMB) void k_mg_cheb(const MgEnt* __restrict__ ents,
   }
 }
 
-// bc = Q^T (b - y), thread per aggregate (members in ascending order)
+// bc = Q^T (b - y): kMG lanes per aggregate stride over its members, then a
+// fixed shuffle tree sums the lanes.
 __global__ __launch_bounds__(kMB) void k_mg_restrict(const MgEnt* __restrict__ ents, const int32_t* __restrict__ map) {
   const MgEnt& E = ents[map[blockIdx.x]];
   if (mg_skip(E)) return;
   const int lb = blockIdx.x - E.ablk0;
-  for (int64_t a = (int64_t)lb * kMB + threadIdx.x; a < E.nagg; a += (int64_t)E.nablk * kMB) {
+  const int l = threadIdx.x % kMG;
+  for (int64_t a = (int64_t)lb * kMSpN + threadIdx.x / kMG; a - threadIdx.x / kMG < E.nagg;
+       a += (int64_t)E.nablk * kMSpN) {
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int32_t m = E.mem_ptr[a]; m < E.mem_ptr[a + 1]; ++m) {
-      const int64_t j = E.mem[m];
-      const float* Qj = E.Q + 18 * j;
+    if (a < E.nagg) {
+      for (int32_t m = E.mem_ptr[a] + l; m < E.mem_ptr[a + 1]; m += kMG) {
+        const int64_t j = E.mem[m];
+        const float* Qj = E.Q + 18 * j;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float r = E.b[3 * j + c] - E.y[3 * j + c];
+        for (int c = 0; c < 3; ++c) {
+          const float r = E.b[3 * j + c] - E.y[3 * j + c];
 #pragma unroll
-        for (int u = 0; u < 6; ++u) acc[u] += Qj[6 * c + u] * r;
+          for (int u = 0; u < 6; ++u) acc[u] += Qj[6 * c + u] * r;
+        }
       }
     }
 #pragma unroll
-    for (int u = 0; u < 6; ++u) E.bc[6 * a + u] = acc[u];
+    for (int o = kMG / 2; o > 0; o >>= 1)
+#pragma unroll
+      for (int u = 0; u < 6; ++u) acc[u] += __shfl_xor(acc[u], o, kMG);
+    if (a < E.nagg && l == 0)
+#pragma unroll
+      for (int u = 0; u < 6; ++u) E.bc[6 * a + u] = acc[u];
   }
 }
 
@@ -1167,7 +1177,7 @@ int amg_sys_create(llfem_ctx* ctx, const AmgHier* h, const float* val32, float* 
     e.mem_ptr = D.mem_ptr;
     e.mem = D.mem;
     e.nagg = D.nagg;
-    e.nablk = (int32_t)cap(D.nagg, kMB, kMaxAB);
+    e.nablk = (int32_t)cap(D.nagg, kMSpN, kMaxAB);
   }
   for (int l = 0; l < L; ++l) {
     s->ent[l].bc = s->ent[l + 1].b;

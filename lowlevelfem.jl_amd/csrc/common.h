@@ -209,6 +209,7 @@ struct SolveSys;
 class BatchSolver {
  public:
   static constexpr double kDelta = 0.1;      // reliable-update trigger (residual reduction)
+  static constexpr double kDeltaAmg = 0.1;   // the same for AMG systems
   static constexpr int kMaxRunAmg = 40;      // AMG systems: FP64 update after at most this many iterations
   // f64_vectors: FP64 Krylov vectors over the FP32 operator (else FP32 vectors)
   BatchSolver(llfem_ctx* c, bool f64_vectors);
@@ -271,7 +272,7 @@ void amg_job_drop(AmgJob* job);                                                 
 
 int64_t sys_iterations(const SolveSys* s);
 int64_t sys_updates(const SolveSys* s);
-int sys_status(const SolveSys* s);      // 0 running, 1 converged, -1 not converged (maxiter)
+int sys_status(const SolveSys* s);      // 0 running, 1 converged, -1 maxiter, -2 non-finite FP32 correction
 double sys_relres(const SolveSys* s);
 
 }  // namespace llfem

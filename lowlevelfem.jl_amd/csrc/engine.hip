@@ -244,11 +244,13 @@ __global__ void k_bs_states(const SysDev* __restrict__ sys, int nsys, double* __
 
 // ---- single-system kernels of the reliable update ------------------------------
 
+// trial solution xt = xh + x / s (xh is kept until xt is known to be finite); x = 0
 template <typename VT>
-__global__ void k_ru_accum(int64_t n, VT* __restrict__ x, double inv_s, double* __restrict__ xh) {
+__global__ void k_ru_trial(int64_t n, VT* __restrict__ x, double inv_s, const double* __restrict__ xh,
+                           double* __restrict__ xt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  xh[i] += (double)x[i] * inv_s;
+  xt[i] = xh[i] + (double)x[i] * inv_s;
   x[i] = (VT)0;
 }
 
@@ -383,15 +385,11 @@ __global__ __launch_bounds__(kSB) void k_ru_restart_amg(int64_t N, int32_t nvblk
   if (threadIdx.x == 0) part[2 * blockIdx.x] = rr;
 }
 
-// AMG restart, second half: p = z on the first start; scalars from the
-// partials (one block).
-__global__ __launch_bounds__(kSB) void k_ru_scal_amg(int64_t n, int32_t nvblk, int init, const float* __restrict__ z,
-                                                     float* __restrict__ p, const double* __restrict__ part,
+// AMG restart, second half: scalars from the partials (one block).
+__global__ __launch_bounds__(kSB) void k_ru_scal_amg(int32_t nvblk, const double* __restrict__ part,
                                                      double* __restrict__ scal, double tol2, double delta2,
                                                      double maxrun) {
   __shared__ double sh[kSB];
-  if (init)
-    for (int64_t i = threadIdx.x; i < n; i += kSB) p[i] = z[i];
   double rr = 0.0, rz = 0.0;
   for (int i = threadIdx.x; i < nvblk; i += kSB) {
     rr += part[2 * i];
@@ -473,13 +471,14 @@ struct SolveSys {
   int64_t n = 0, nadj = 0;
   void* mem = nullptr;
   size_t bytes = 0;
-  double *b = nullptr, *xh = nullptr, *rh = nullptr, *y = nullptr, *scal = nullptr, *part = nullptr;
+  double *b = nullptr, *xh = nullptr, *xt = nullptr, *rh = nullptr, *y = nullptr, *scal = nullptr, *part = nullptr;
+  int bad = 0;              // consecutive updates whose FP32 correction was not finite
   void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;   // Krylov vectors (VT)
   float *binv = nullptr, *val32 = nullptr;
   int64_t npart = 0;
   double bnorm = 0.0, scale = 1.0, relres = 0.0;
   int64_t iters = 0, updates = 0;
-  int status = 0;           // 0 running, 1 converged, -1 failed
+  int status = 0;           // 0 running, 1 converged, -1 maxiter, -2 non-finite FP32 corrections
   int32_t sblk0 = 0, nsblk = 0, vblk0 = 0, nvblk = 0;
   AmgSys* amg = nullptr;
 };
@@ -503,9 +502,9 @@ size_t BatchSolver::bytes_needed(int64_t N, int64_t nadj, bool f64_vectors) {
   const int64_t n = 3 * N;
   const int64_t nsblk = spmv_blocks(N), nvblk = vec_blocks(N);
   const int64_t npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
-  return sizeof(double) * (4 * (size_t)n + SC_N + (size_t)npart) +
+  return sizeof(double) * (5 * (size_t)n + SC_N + (size_t)npart) +
          (f64_vectors ? sizeof(double) : sizeof(float)) * 5 * (size_t)n +
-         sizeof(float) * (9 * (size_t)N + 9 * (size_t)nadj) + 16 * 256;   // 13 buffers, 256-B aligned
+         sizeof(float) * (9 * (size_t)N + 9 * (size_t)nadj) + 16 * 256;   // 14 buffers, 256-B aligned
 }
 
 // Reliable update of one system: fold x32 into xh, recompute the FP64
@@ -514,23 +513,51 @@ int BatchSolver::update(SolveSys* s, bool init) {
   hipStream_t st = ctx_->stream;
   const SysSpec& sp = s->spec;
   const int64_t n = s->n;
-  if (!init) {
-    if (f64v_) k_ru_accum<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh);
-    else k_ru_accum<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh);
-    k_spmv64<<<blocks(sp.N * kSG), kSB, 0, st>>>(sp.N, sp.adj_ptr, sp.adj, sp.val64, s->xh, sp.fixed, s->y);
-    k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, s->y, sp.fixed, s->rh, s->part);
-  } else {
-    k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, nullptr, sp.fixed, s->rh, s->part);
-  }
-  k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 1);
+  // r = b - K x in FP64 for x (NULL: x = 0); r2 = ||r||^2 read back
+  auto residual = [&](const double* x, double* r2) -> int {
+    if (x) {
+      k_spmv64<<<blocks(sp.N * kSG), kSB, 0, st>>>(sp.N, sp.adj_ptr, sp.adj, sp.val64, x, sp.fixed, s->y);
+      k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, s->y, sp.fixed, s->rh, s->part);
+    } else {
+      k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, nullptr, sp.fixed, s->rh, s->part);
+    }
+    k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 1);
+    LLFEM_HIP(hipMemcpyAsync(r2, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  };
   double r2 = 0.0, it = 0.0;
-  LLFEM_HIP(hipMemcpyAsync(&r2, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  bool fresh = init;   // restart the Krylov recursion from scratch (p = z)
+  if (!init) {
+    if (f64v_) k_ru_trial<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh, s->xt);
+    else k_ru_trial<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh, s->xt);
+    LLFEM_TRY(residual(s->xt, &r2));
+    if (std::isfinite(r2)) {
+      std::swap(s->xh, s->xt);
+      s->bad = 0;
+    } else {
+      // the FP32 correction overflowed: keep xh, restart the recursion from it
+      ++s->bad;
+      fresh = true;
+      LLFEM_TRY(residual(s->xh, &r2));
+    }
+  } else {
+    LLFEM_TRY(residual(nullptr, &r2));
+  }
   LLFEM_HIP(hipMemcpyAsync(&it, s->scal + SC_ITER, sizeof(double), hipMemcpyDeviceToHost, st));
   LLFEM_HIP(hipStreamSynchronize(st));
   const double rn = std::sqrt(r2);
   s->iters = (int64_t)it;
   if (init) s->bnorm = rn;
   else ++s->updates;
+  if (s->bad >= 3 || !std::isfinite(rn)) {
+    s->status = -2;
+    s->relres = s->bnorm > 0 && std::isfinite(rn) ? rn / s->bnorm : 0.0;
+    const double idle = ST_IDLE;
+    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  }
   if (rn <= sp.reltol * s->bnorm || s->bnorm == 0.0) {
     s->status = 1;
     const double idle = ST_IDLE;
@@ -553,20 +580,20 @@ int BatchSolver::update(SolveSys* s, bool init) {
   if (s->amg) {
     const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
     double* vpart = s->part + nsblk;
-    k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, init ? 1 : 0, (float*)s->r,
+    k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, fresh ? 1 : 0, (float*)s->r,
                                                       (float*)s->p, vpart);
     LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
-    k_ru_scal_amg<<<1, kSB, 0, st>>>(n, nvblk, init ? 1 : 0, (const float*)s->z, (float*)s->p, vpart, s->scal,
-                                     tol * tol, kDelta * kDelta, (double)kMaxRunAmg);
+    if (fresh) LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
     LLFEM_HIP(hipGetLastError());
     return LLFEM_OK;
   }
   const int64_t nb = blocks(sp.N);
   if (f64v_)
-    k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, init ? 1 : 0, s->binv,
+    k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, fresh ? 1 : 0, s->binv,
                                                        (double*)s->r, (double*)s->z, (double*)s->p, s->part);
   else
-    k_ru_restart<float><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, init ? 1 : 0, s->binv,
+    k_ru_restart<float><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, fresh ? 1 : 0, s->binv,
                                                       (float*)s->r, (float*)s->z, (float*)s->p, s->part);
   k_ru_scal<<<1, kSB, 0, st>>>(nb, s->part, s->scal, tol * tol, kDelta * kDelta);
   LLFEM_HIP(hipGetLastError());
@@ -596,6 +623,7 @@ int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
   };
   s->b = (double*)take(sizeof(double) * n);
   s->xh = (double*)take(sizeof(double) * n);
+  s->xt = (double*)take(sizeof(double) * n);
   s->rh = (double*)take(sizeof(double) * n);
   s->y = (double*)take(sizeof(double) * n);
   s->scal = (double*)take(sizeof(double) * SC_N);

@@ -376,7 +376,9 @@ void complete(llfem_server* S, Req* q) {
   q->iterations = sys_iterations(q->sys);
   q->updates = sys_updates(q->sys);
   q->relres = sys_relres(q->sys);
-  if (rc == LLFEM_OK && status != 1)
+  if (rc == LLFEM_OK && status == -2)
+    rc = fail(LLFEM_ENOCONV, "cg: the FP32 correction was not finite in three successive updates");
+  else if (rc == LLFEM_OK && status != 1)
     rc = fail(LLFEM_ENOCONV, "cg: maxiter reached (relative residual " + std::to_string(q->relres) + ")");
   if (rc != LLFEM_OK) q->err = llfem_last_error();
   q->eng->remove(q->sys);

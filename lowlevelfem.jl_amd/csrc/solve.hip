@@ -656,6 +656,8 @@ int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const doub
   }
   const double rr = sys_relres(s);
   eng.remove(s);
+  if (status == -2)
+    return fail(LLFEM_ENOCONV, "cg: the FP32 correction was not finite in three successive updates");
   if (status != 1)
     return fail(LLFEM_ENOCONV, "cg: maxiter reached (relative residual " + std::to_string(rr) + ")");
   return LLFEM_OK;
