@@ -385,6 +385,34 @@ __global__ __launch_bounds__(kSB) void k_ru_restart_amg(int64_t N, int32_t nvblk
   if (threadIdx.x == 0) part[2 * blockIdx.x] = rr;
 }
 
+// AMG restart without a V-cycle: r = rh * s, and p, z rescaled by pscale.
+// The true residual differs from the rescaled recursive one only by the FP32
+// drift since the last update, so z = M r carries over with the same scale.
+__global__ __launch_bounds__(kSB) void k_ru_rescale_amg(int64_t N, int32_t nvblk, const double* __restrict__ rh,
+                                                        double s, float pscale, float* __restrict__ r32,
+                                                        float* __restrict__ z32, float* __restrict__ p32,
+                                                        double* __restrict__ part) {
+  __shared__ double sh[kSB];
+  double rr = 0.0, rz = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kVecNodes + threadIdx.x; j < N; j += (int64_t)nvblk * kVecNodes)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float r = (float)(rh[3 * j + c] * s);
+      const float z = pscale * z32[3 * j + c];
+      r32[3 * j + c] = r;
+      z32[3 * j + c] = z;
+      p32[3 * j + c] = pscale * p32[3 * j + c];
+      rr += (double)r * (double)r;
+      rz += (double)r * (double)z;
+    }
+  rr = bsum(rr, sh);
+  rz = bsum(rz, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = rr;
+    part[2 * blockIdx.x + 1] = rz;
+  }
+}
+
 // AMG restart, second half: scalars from the partials (one block).
 __global__ __launch_bounds__(kSB) void k_ru_scal_amg(int32_t nvblk, const double* __restrict__ part,
                                                      double* __restrict__ scal, double tol2, double delta2,
@@ -580,10 +608,15 @@ int BatchSolver::update(SolveSys* s, bool init) {
   if (s->amg) {
     const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
     double* vpart = s->part + nsblk;
-    k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, fresh ? 1 : 0, (float*)s->r,
-                                                      (float*)s->p, vpart);
-    LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
-    if (fresh) LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    if (fresh) {   // z = M r by a V-cycle of this system alone, p = z
+      k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, 1, (float*)s->r,
+                                                        (float*)s->p, vpart);
+      LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
+      LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    } else {
+      k_ru_rescale_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, (float)pscale, (float*)s->r,
+                                                        (float*)s->z, (float*)s->p, vpart);
+    }
     k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
     LLFEM_HIP(hipGetLastError());
     return LLFEM_OK;
