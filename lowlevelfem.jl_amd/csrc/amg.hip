@@ -104,24 +104,34 @@ namespace {
 
 __device__ __forceinline__ bool mg_skip(const MgEnt& E) { return E.state && *E.state != 0.0; }
 
+
 // y = mask(A x) for every entry's level
 __global__ __launch_bounds__(kMB) void k_mg_spmv(const MgEnt* __restrict__ ents, const int32_t* __restrict__ map) {
   const MgEnt& E = ents[map[blockIdx.x]];
   if (mg_skip(E)) return;
+  // local restrict-qualified copies: the output never aliases the inputs
+  const int64_t N = E.N;
+  const auto* __restrict__ adj_ptr = gp(E.adj_ptr);
+  const auto* __restrict__ adj = gp(E.adj);
+  const auto* __restrict__ val = gp(E.val);
+  const auto* __restrict__ x = gp(E.x);
+  const auto* __restrict__ mask = gp(E.mask);
+  auto* __restrict__ y = gp(E.y);
   const int lb = blockIdx.x - E.sblk0;
+  const int nsblk = E.nsblk;
   const int l = threadIdx.x % kMG;
-  for (int64_t j = (int64_t)lb * kMSpN + threadIdx.x / kMG; j - threadIdx.x / kMG < E.N;
-       j += (int64_t)E.nsblk * kMSpN) {
+  for (int64_t j = (int64_t)lb * kMSpN + threadIdx.x / kMG; j - threadIdx.x / kMG < N;
+       j += (int64_t)nsblk * kMSpN) {
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    if (j < E.N) {
-      const int64_t p0 = E.adj_ptr[j];
-      const int R = (int)(E.adj_ptr[j + 1] - p0);
-      const float* P = E.val + 9 * p0;
+    if (j < N) {
+      const int64_t p0 = adj_ptr[j];
+      const int R = (int)(adj_ptr[j + 1] - p0);
+      const auto* P = val + 9 * p0;
 #pragma unroll 2
       for (int t = l; t < R; t += kMG) {
-        const int64_t i = E.adj[p0 + t];
-        const float x0 = E.x[3 * i], x1 = E.x[3 * i + 1], x2 = E.x[3 * i + 2];
-        const float* B = P + 9 * t;
+        const int64_t i = adj[p0 + t];
+        const float x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+        const auto* B = P + 9 * t;
         a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
         a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
         a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
@@ -133,10 +143,10 @@ __global__ __launch_bounds__(kMB) void k_mg_spmv(const MgEnt* __restrict__ ents,
       a1 += __shfl_xor(a1, o, kMG);
       a2 += __shfl_xor(a2, o, kMG);
     }
-    if (j < E.N && l == 0) {
-      E.y[3 * j] = E.mask[3 * j] ? 0.f : a0;
-      E.y[3 * j + 1] = E.mask[3 * j + 1] ? 0.f : a1;
-      E.y[3 * j + 2] = E.mask[3 * j + 2] ? 0.f : a2;
+    if (j < N && l == 0) {
+      y[3 * j] = mask[3 * j] ? 0.f : a0;
+      y[3 * j + 1] = mask[3 * j + 1] ? 0.f : a1;
+      y[3 * j + 2] = mask[3 * j + 2] ? 0.f : a2;
     }
   }
 }
@@ -146,10 +156,15 @@ __global__ __launch_bounds__(kMB) void k_mg_cheb0(const MgEnt* __restrict__ ents
   const MgEnt& E = ents[map[blockIdx.x]];
   if (mg_skip(E)) return;
   const int lb = blockIdx.x - E.vblk0;
+  const float* __restrict__ dinv = (const float*)gp(E.dinv);
+  const float* __restrict__ b = (const float*)gp(E.b);
+  float* __restrict__ x = (float*)gp(E.x);
+  float* __restrict__ d = (float*)gp(E.d);
+  const float it = E.inv_theta;
   for (int64_t i = (int64_t)lb * kMB + threadIdx.x; i < 3 * E.N; i += (int64_t)E.nvblk * kMB) {
-    const float v = E.inv_theta * E.dinv[i] * E.b[i];
-    E.x[i] = v;
-    E.d[i] = v;
+    const float v = it * dinv[i] * b[i];
+    x[i] = v;
+    d[i] = v;
   }
 }
 
@@ -162,18 +177,24 @@ __global__ __launch_bounds__(kMB) void k_mg_cheb(const MgEnt* __restrict__ ents,
   if (mg_skip(E)) return;
   const int lb = blockIdx.x - E.vblk0;
   const float cb = cbt * E.inv_theta + cbd * E.inv_delta;
+  const float* __restrict__ b = (const float*)gp(E.b);
+  const float* __restrict__ y = (const float*)gp(E.y);
+  const float* __restrict__ dinv = (const float*)gp(E.dinv);
+  float* __restrict__ d = (float*)gp(E.d);
+  float* __restrict__ x = (float*)gp(E.x);
+  const int64_t n = 3 * E.N;
   double acc = 0.0;
-  for (int64_t i = (int64_t)lb * kMB + threadIdx.x; i < 3 * E.N; i += (int64_t)E.nvblk * kMB) {
-    const float bi = E.b[i];
-    const float dv = ca * E.d[i] + cb * E.dinv[i] * (bi - E.y[i]);
-    const float xv = E.x[i] + dv;
-    E.d[i] = dv;
-    E.x[i] = xv;
+  for (int64_t i = (int64_t)lb * kMB + threadIdx.x; i < n; i += (int64_t)E.nvblk * kMB) {
+    const float bi = b[i];
+    const float dv = ca * d[i] + cb * dinv[i] * (bi - y[i]);
+    const float xv = x[i] + dv;
+    d[i] = dv;
+    x[i] = xv;
     acc += (double)bi * (double)xv;
   }
   if (rz) {
     acc = msum(acc, sh);
-    if (threadIdx.x == 0) E.rzpart[2 * lb + 1] = acc;
+    if (threadIdx.x == 0) ((double*)gp(E.rzpart))[2 * lb + 1] = acc;
   }
 }
 
@@ -184,16 +205,22 @@ __global__ __launch_bounds__(kMB) void k_mg_restrict(const MgEnt* __restrict__ e
   if (mg_skip(E)) return;
   const int lb = blockIdx.x - E.ablk0;
   const int l = threadIdx.x % kMG;
+  const int32_t* __restrict__ mem_ptr = (const int32_t*)gp(E.mem_ptr);
+  const int32_t* __restrict__ mem = (const int32_t*)gp(E.mem);
+  const float* __restrict__ Q = (const float*)gp(E.Q);
+  const float* __restrict__ b = (const float*)gp(E.b);
+  const float* __restrict__ y = (const float*)gp(E.y);
+  float* __restrict__ bc = (float*)gp(E.bc);
   for (int64_t a = (int64_t)lb * kMSpN + threadIdx.x / kMG; a - threadIdx.x / kMG < E.nagg;
        a += (int64_t)E.nablk * kMSpN) {
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (a < E.nagg) {
-      for (int32_t m = E.mem_ptr[a] + l; m < E.mem_ptr[a + 1]; m += kMG) {
-        const int64_t j = E.mem[m];
-        const float* Qj = E.Q + 18 * j;
+      for (int32_t m = mem_ptr[a] + l; m < mem_ptr[a + 1]; m += kMG) {
+        const int64_t j = mem[m];
+        const float* Qj = Q + 18 * j;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const float r = E.b[3 * j + c] - E.y[3 * j + c];
+          const float r = b[3 * j + c] - y[3 * j + c];
 #pragma unroll
           for (int u = 0; u < 6; ++u) acc[u] += Qj[6 * c + u] * r;
         }
@@ -205,7 +232,7 @@ __global__ __launch_bounds__(kMB) void k_mg_restrict(const MgEnt* __restrict__ e
       for (int u = 0; u < 6; ++u) acc[u] += __shfl_xor(acc[u], o, kMG);
     if (a < E.nagg && l == 0)
 #pragma unroll
-      for (int u = 0; u < 6; ++u) E.bc[6 * a + u] = acc[u];
+      for (int u = 0; u < 6; ++u) bc[6 * a + u] = acc[u];
   }
 }
 
@@ -215,11 +242,15 @@ __global__ __launch_bounds__(kMB) void k_mg_prolong(const MgEnt* __restrict__ en
   const MgEnt& E = ents[map[blockIdx.x]];
   if (mg_skip(E)) return;
   const int lb = blockIdx.x - E.vblk0;
+  const int32_t* __restrict__ agg = (const int32_t*)gp(E.agg);
+  const float* __restrict__ Q = (const float*)gp(E.Q);
+  const float* __restrict__ xcv = (const float*)gp(E.xc);
+  float* __restrict__ x = (float*)gp(E.x);
   for (int64_t j = (int64_t)lb * kMB + threadIdx.x; j < E.N; j += (int64_t)E.nvblk * kMB) {
-    const int32_t a = E.agg[j];
+    const int32_t a = agg[j];
     if (a < 0) continue;
-    const float* Qj = E.Q + 18 * j;
-    const float* xc = E.xc + 6 * (int64_t)a;
+    const float* Qj = Q + 18 * j;
+    const float* xc = xcv + 6 * (int64_t)a;
     float v[6];
 #pragma unroll
     for (int u = 0; u < 6; ++u) v[u] = xc[u];
@@ -228,7 +259,7 @@ __global__ __launch_bounds__(kMB) void k_mg_prolong(const MgEnt* __restrict__ en
       float s = 0.f;
 #pragma unroll
       for (int u = 0; u < 6; ++u) s += Qj[6 * c + u] * v[u];
-      E.x[3 * j + c] += alpha * s;
+      x[3 * j + c] += alpha * s;
     }
   }
 }
@@ -238,11 +269,14 @@ __global__ __launch_bounds__(kMB) void k_mg_coarse(const MgEnt* __restrict__ ent
   const MgEnt& E = ents[blockIdx.x];
   if (mg_skip(E)) return;
   const int n = E.nc;
+  const float* __restrict__ Ainv = (const float*)gp(E.Ainv);
+  const float* __restrict__ b = (const float*)gp(E.b);
+  float* __restrict__ x = (float*)gp(E.x);
   for (int i = threadIdx.x; i < n; i += kMB) {
-    const float* row = E.Ainv + (int64_t)i * n;
+    const float* row = Ainv + (int64_t)i * n;
     float s = 0.f;
-    for (int k = 0; k < n; ++k) s += row[k] * E.b[k];
-    E.x[i] = s;
+    for (int k = 0; k < n; ++k) s += row[k] * b[k];
+    x[i] = s;
   }
 }
 

@@ -117,7 +117,8 @@ __global__ void k_geometry_batch(const GeoSeg* __restrict__ segs, int nseg, int6
     if (segs[mid].e0 <= t) lo = mid; else hi = mid - 1;
   }
   const GeoSeg& g = segs[lo];
-  geometry_one(g.E, g.N, g.npe, g.q, g.affine, g.xyz, g.conn, g.dN, g.w, g.geo, t - g.e0);
+  geometry_one(g.E, g.N, g.npe, g.q, g.affine, (const double*)gp(g.xyz), (const int32_t*)gp(g.conn),
+               (const double*)gp(g.dN), (const double*)gp(g.w), (double*)gp(g.geo), t - g.e0);
 }
 
 __global__ void k_zero_u32(uint32_t* p) {

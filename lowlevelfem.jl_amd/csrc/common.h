@@ -32,6 +32,16 @@ void set_error(const std::string& msg);
     if (_s != LLFEM_OK) return _s; \
   } while (0)
 
+// Pointers read from device-memory tables are generic to the compiler, which
+// then emits flat loads and stores (counted against both vmcnt and lgkmcnt,
+// so every wait becomes a full drain).  Every such pointer here addresses
+// device memory: casting it to the global address space restores global_*
+// instructions.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gp(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 inline int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;

@@ -86,8 +86,12 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
   __shared__ double sh[kSB];
   const SysDev& S = sys[bsys[blockIdx.x]];
   if (S.scal[SC_STATE] != ST_RUN) return;   // block-uniform
-  const VT* __restrict__ pv = static_cast<const VT*>(S.p);
-  VT* __restrict__ qv = static_cast<VT*>(S.q);
+  const VT* __restrict__ pv = (const VT*)gp(static_cast<const VT*>(S.p));
+  VT* __restrict__ qv = (VT*)gp(static_cast<VT*>(S.q));
+  const int64_t* __restrict__ adj_ptr = (const int64_t*)gp(S.adj_ptr);
+  const int32_t* __restrict__ adj = (const int32_t*)gp(S.adj);
+  const float* __restrict__ val = (const float*)gp(S.val);
+  const uint8_t* __restrict__ fixed = (const uint8_t*)gp(S.fixed);
   const int lb = blockIdx.x - S.sblk0;
   const int l = threadIdx.x % kSG;
   double d = 0.0;
@@ -95,12 +99,12 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
        j += (int64_t)S.nsblk * kSpmvNodes) {
   VT a0 = 0, a1 = 0, a2 = 0;
   if (j < S.N) {
-    const int64_t p0 = S.adj_ptr[j];
-    const int R = (int)(S.adj_ptr[j + 1] - p0);
-    const float* P = S.val + 9 * p0;
+    const int64_t p0 = adj_ptr[j];
+    const int R = (int)(adj_ptr[j + 1] - p0);
+    const float* P = val + 9 * p0;
 #pragma unroll 2
     for (int t = l; t < R; t += kSG) {
-      const int64_t i = S.adj[p0 + t];
+      const int64_t i = adj[p0 + t];
       const VT x0 = pv[3 * i], x1 = pv[3 * i + 1], x2 = pv[3 * i + 2];
       const float* B = P + 9 * t;   // block t: [c][d], 36 contiguous bytes
       a0 += (VT)B[0] * x0 + (VT)B[1] * x1 + (VT)B[2] * x2;
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(kSB) void k_bs_spmv(const SysDev* __restrict__ sys,
     const VT a[3] = {a0, a1, a2};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const VT v = S.fixed[3 * j + c] ? (VT)0 : a[c];
+      const VT v = fixed[3 * j + c] ? (VT)0 : a[c];
       qv[3 * j + c] = v;
       d += (double)pv[3 * j + c] * (double)v;
     }
@@ -159,11 +163,12 @@ __global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sy
   __shared__ double sh[kSB];
   const SysDev& S = sys[bsys[blockIdx.x]];
   if (S.scal[SC_STATE] != ST_RUN) return;
-  VT* __restrict__ xv = static_cast<VT*>(S.x);
-  VT* __restrict__ rv = static_cast<VT*>(S.r);
-  VT* __restrict__ zv = static_cast<VT*>(S.z);
-  const VT* __restrict__ pv = static_cast<const VT*>(S.p);
-  const VT* __restrict__ qv = static_cast<const VT*>(S.q);
+  VT* __restrict__ xv = (VT*)gp(static_cast<VT*>(S.x));
+  VT* __restrict__ rv = (VT*)gp(static_cast<VT*>(S.r));
+  VT* __restrict__ zv = (VT*)gp(static_cast<VT*>(S.z));
+  const VT* __restrict__ pv = (const VT*)gp(static_cast<const VT*>(S.p));
+  const VT* __restrict__ qv = (const VT*)gp(static_cast<const VT*>(S.q));
+  const float* __restrict__ binv = (const float*)gp(S.binv);
   const int lb = blockIdx.x - S.vblk0;
   double rr = 0.0, rz = 0.0;
   const VT alpha = (VT)S.scal[SC_ALPHA];
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(kSB) void k_bs_update(const SysDev* __restrict__ sy
 #pragma unroll
     for (int c = 0; c < 3; ++c) rr += (double)r[c] * (double)r[c];
     if (S.amg) continue;
-    const float* B = S.binv + 9 * j;
+    const float* B = binv + 9 * j;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const VT z = (VT)B[3 * c] * r[0] + (VT)B[3 * c + 1] * r[1] + (VT)B[3 * c + 2] * r[2];
@@ -226,8 +231,8 @@ __global__ __launch_bounds__(kSB) void k_bs_pupd(const SysDev* __restrict__ sys,
   const SysDev& S = sys[bsys[blockIdx.x]];
   const double st = S.scal[SC_STATE];
   if (st != ST_RUN && st != ST_PENDING) return;
-  VT* __restrict__ pv = static_cast<VT*>(S.p);
-  const VT* __restrict__ zv = static_cast<const VT*>(S.z);
+  VT* __restrict__ pv = (VT*)gp(static_cast<VT*>(S.p));
+  const VT* __restrict__ zv = (const VT*)gp(static_cast<const VT*>(S.z));
   const int lb = blockIdx.x - S.vblk0;
   const VT beta = (VT)S.scal[SC_BETA];
   for (int64_t j = (int64_t)lb * kVecNodes + threadIdx.x; j < S.N; j += (int64_t)S.nvblk * kVecNodes)

@@ -252,12 +252,12 @@ struct TabBlk {
 
 // Operator-table entry of (a, b): Ghat (9 values) for stiffness, Mhat for mass.
 template <int NPE, int OP, bool AFFINE, typename Real>
-__device__ __forceinline__ TabBlk<Real> load_tab(const PanelParams& P, int a, int b) {
+__device__ __forceinline__ TabBlk<Real> load_tab(const double* tab, const float* tabf, int a, int b) {
   TabBlk<Real> t;
   if constexpr (AFFINE && OP == LLFEM_OP_STIFFNESS) {
-    load9<Real>(P.tab + (size_t)(a * NPE + b) * 9, P.tabf + (size_t)(a * NPE + b) * 9, t.v);
+    load9<Real>(tab + (size_t)(a * NPE + b) * 9, tabf + (size_t)(a * NPE + b) * 9, t.v);
   } else if constexpr (AFFINE) {
-    t.v[0] = (Real)P.tab[a * NPE + b];
+    t.v[0] = (Real)tab[a * NPE + b];
   }
   return t;
 }
@@ -265,9 +265,9 @@ __device__ __forceinline__ TabBlk<Real> load_tab(const PanelParams& P, int a, in
 // Add element block Ke[(b,:),(a,:)] to the panel rows of slot s (or, for a
 // non-affine element, integrate it point by point first).
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
-__device__ __forceinline__ void contribute(const PanelParams& P, const TabBlk<Real>& tb,
-                                           double* __restrict__ wl, int base, int R, int b, int e,
-                                           int a, int s, const Geo<Real>& g) {
+__device__ __forceinline__ void contribute(const PanelParams& P, const double* geo, const double* tab,
+                                           const TabBlk<Real>& tb, double* __restrict__ wl, int base, int R,
+                                           int b, int e, int a, int s, const Geo<Real>& g) {
   if constexpr (OP == LLFEM_OP_STIFFNESS) {
     Real G[9];
     Real scale;
@@ -279,14 +279,14 @@ __device__ __forceinline__ void contribute(const PanelParams& P, const TabBlk<Re
       for (int m = 0; m < 9; ++m) G[m] = 0;
       scale = 1;
       for (int p = 0; p < P.q; ++p) {
-        const Geo<Real> gp = load_geo<Real>(P.geo + ((size_t)e * P.q + p) * kGeoStride);
-        const double* hb = P.tab + ((size_t)p * NPE + b) * 3;
-        const double* ha = P.tab + ((size_t)p * NPE + a) * 3;
+        const Geo<Real> gq = load_geo<Real>(geo + ((size_t)e * P.q + p) * kGeoStride);
+        const double* hb = tab + ((size_t)p * NPE + b) * 3;
+        const double* ha = tab + ((size_t)p * NPE + a) * 3;
         Real gb[3], ga[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          gb[i] = gp.Mi[i] * (Real)hb[0] + gp.Mi[3 + i] * (Real)hb[1] + gp.Mi[6 + i] * (Real)hb[2];
-          ga[i] = (gp.Mi[i] * (Real)ha[0] + gp.Mi[3 + i] * (Real)ha[1] + gp.Mi[6 + i] * (Real)ha[2]) * gp.wd;
+          gb[i] = gq.Mi[i] * (Real)hb[0] + gq.Mi[3 + i] * (Real)hb[1] + gq.Mi[6 + i] * (Real)hb[2];
+          ga[i] = (gq.Mi[i] * (Real)ha[0] + gq.Mi[3 + i] * (Real)ha[1] + gq.Mi[6 + i] * (Real)ha[2]) * gq.wd;
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -314,7 +314,7 @@ __device__ __forceinline__ void contribute(const PanelParams& P, const TabBlk<Re
     } else {
       mv = 0.0;
       for (int p = 0; p < P.q; ++p)
-        mv += P.geo[((size_t)e * P.q + p) * kGeoStride + 9] * P.tab[p * NPE + b] * P.tab[p * NPE + a];
+        mv += geo[((size_t)e * P.q + p) * kGeoStride + 9] * tab[p * NPE + b] * tab[p * NPE + a];
       mv *= P.rho;
     }
     lds_add(wl + base + s, mv);
@@ -347,11 +347,11 @@ struct StepRec {
 };
 
 template <int NPE, typename MapT>
-__device__ __forceinline__ StepRec load_rec(const PanelParams& P, int64_t r, int b) {
+__device__ __forceinline__ StepRec load_rec(const void* recs, int rec_bytes, int64_t r, int b) {
   StepRec o;
   if constexpr (NPE <= 11 && sizeof(MapT) == 1) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(P.rec) + r);
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(recs) + r);
     o.e = (int)v.x;
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
     const int byte = 5 + b;
@@ -359,7 +359,7 @@ __device__ __forceinline__ StepRec load_rec(const PanelParams& P, int64_t r, int
     o.a = a8 == 0xffu ? -1 : (int)a8;
     o.s = (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
   } else {
-    const uint8_t* base = static_cast<const uint8_t*>(P.rec) + r * P.rec_bytes;
+    const uint8_t* base = static_cast<const uint8_t*>(recs) + r * rec_bytes;
     const int2 h = *reinterpret_cast<const int2*>(base);
     o.e = h.x;
     o.a = (int)(int16_t)(h.y & 0xffff);
@@ -390,8 +390,17 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
 
   for (int i = lane; i < NPW * P.max_r * BLK + 9; i += 64) wl[i] = 0.0;
 
+  // device-memory pointers, cast to the global address space (when P lives in
+  // a device-memory table the compiler cannot infer it and would emit flat
+  // loads and stores)
+  const void* const recs = (const void*)gp(P.rec);
+  const double* const geo = (const double*)gp(P.geo);
+  const double* const tab = (const double*)gp(P.tab);
+  const float* const tabf = (const float*)gp(P.tabf);
+  double* const out = (double*)gp(P.out);
+
   // node descriptors of the current and next round
-  const int2* desc = P.desc + w0 * (int64_t)P.rounds * NPW;
+  const int2* desc = (const int2*)gp(P.desc) + w0 * (int64_t)P.rounds * NPW;
   int2 dcur = desc[lane_rec];
   int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
 
@@ -400,11 +409,11 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
   TabBlk<Real> tb[PF];
 #pragma unroll
   for (int v = 0; v < 2 * PF; ++v)
-    rec[v] = load_rec<NPE, MapT>(P, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
+    rec[v] = load_rec<NPE, MapT>(recs, P.rec_bytes, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
 #pragma unroll
   for (int v = 0; v < PF; ++v) {
-    g[v] = load_geo<Real>(P.geo + (size_t)rec[v].e * kGeoStride);
-    tb[v] = load_tab<NPE, OP, AFFINE, Real>(P, rec[v].a < 0 ? 0 : rec[v].a, b);
+    g[v] = load_geo<Real>(geo + (size_t)rec[v].e * kGeoStride);
+    tb[v] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, rec[v].a < 0 ? 0 : rec[v].a, b);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -417,7 +426,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
       if (s < nsteps) {
         const StepRec& cr = rec[u];
         const bool live = grp && cr.a >= 0 && dcur.y > 0;
-        contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, tb[u % PF], live ? pan : dummy,
+        contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, geo, tab, tb[u % PF], live ? pan : dummy,
                                                      0, live ? dcur.y : 0, b, cr.e,
                                                      cr.a < 0 ? 0 : cr.a, live ? cr.s : 0,
                                                      g[u % PF]);
@@ -431,13 +440,13 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
             const int64_t om = (int64_t)__shfl(dcur.x, m * NPE);
             double* pm = wl + (size_t)m * P.max_r * BLK;
             if constexpr (OP == LLFEM_OP_STIFFNESS) {
-              double* o = P.out + 9 * om;
+              double* o = out + 9 * om;
               for (int i = lane; i < 9 * Rm; i += 64) {
                 __builtin_nontemporal_store(pm[i], o + i);
                 pm[i] = 0.0;
               }
             } else {
-              double* o = P.out + 3 * om;
+              double* o = out + 3 * om;
               for (int i = lane; i < 3 * Rm; i += 64) {
                 const int c = i / Rm;
                 __builtin_nontemporal_store(pm[i - c * Rm], o + i);
@@ -458,9 +467,9 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
       }
       // ring refill: geometry/table for step s+PF, record for step s+2PF
       const StepRec& nx = rec[(u + PF) % (2 * PF)];
-      g[u % PF] = load_geo<Real>(P.geo + (size_t)nx.e * kGeoStride);
-      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(P, nx.a < 0 ? 0 : nx.a, b);
-      rec[u] = load_rec<NPE, MapT>(P, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
+      g[u % PF] = load_geo<Real>(geo + (size_t)nx.e * kGeoStride);
+      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, nx.a < 0 ? 0 : nx.a, b);
+      rec[u] = load_rec<NPE, MapT>(recs, P.rec_bytes, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
     }
   }
 }
