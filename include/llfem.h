@@ -160,9 +160,13 @@ LLFEM_API int llfem_load_traction(llfem_mesh* mesh, int32_t face_type, int32_t n
  * Replaces solveDisplacement(K, f; support, iterative=true, reltol, maxiter,
  * preconditioner) (src/linear.jl:3711-3743) on the stiffness most recently
  * assembled on `mesh`: u_fixed on the constrained DoFs, rhs = f - K u_fixed
- * on the free DoFs, K_ff u_free = rhs by Jacobi-PCG.  fixed[] lists the
- * constrained DoFs (constrainedDoFs, src/general.jl:5289-5322) and
- * u_fixed[] their values (NULL = zeros).  u_out is [pdim*N]. */
+ * on the free DoFs, K_ff u_free = rhs by PCG (method: LLFEM_PCG_*; the
+ * reference's `preconditioner` keyword, passed to cg as Pl at :3736, maps
+ * to the method: Jacobi, or LLFEM_PCG_AMG_RELIABLE for the aggregation-AMG
+ * V-cycle whose hierarchy is kept on the mesh per support; stats->setup_ms
+ * reports its set-up).  fixed[] lists the constrained DoFs (constrainedDoFs,
+ * src/general.jl:5289-5322) and u_fixed[] their values (NULL = zeros).
+ * u_out is [pdim*N]. */
 LLFEM_API int llfem_solve(llfem_mesh* mesh, const double* f, const int64_t* fixed, int64_t nfixed,
                 int32_t one_based, const double* u_fixed, int32_t method, double reltol,
                 int64_t maxiter, double* u_out, llfem_stats* stats);
