@@ -52,7 +52,7 @@ int npe_index(int npe) {
   }
 }
 
-constexpr int batch_kind(int npe_idx, int op, bool affine) { return (npe_idx * 2 + op) * 2 + (affine ? 1 : 0); }
+constexpr int batch_kind(int npe_idx, int op, bool affine) { return (npe_idx * 3 + op) * 2 + (affine ? 1 : 0); }
 
 template <int NPE, int OP, bool AFFINE>
 __device__ __forceinline__ void run_item(const PanelParams& P, int64_t w0, int nw, double* wl) {
@@ -102,6 +102,10 @@ __global__ __launch_bounds__(256) void k_panel_batch(const BatchSeg* __restrict_
       case batch_kind(3, 0, true): run_item<10, 0, true>(P, w, nw, wl); break;
       case batch_kind(3, 1, false): run_item<10, 1, false>(P, w, nw, wl); break;
       case batch_kind(3, 1, true): run_item<10, 1, true>(P, w, nw, wl); break;
+      case batch_kind(0, kOpKM, true): run_item<4, kOpKM, true>(P, w, nw, wl); break;
+      case batch_kind(1, kOpKM, true): run_item<6, kOpKM, true>(P, w, nw, wl); break;
+      case batch_kind(2, kOpKM, true): run_item<8, kOpKM, true>(P, w, nw, wl); break;
+      case batch_kind(3, kOpKM, true): run_item<10, kOpKM, true>(P, w, nw, wl); break;
       default: break;
     }
   }
@@ -184,6 +188,8 @@ struct Item {
   bool affine;
   const double* rule;   // device [dN | w]
   const double* tab;    // device operator table
+  int mass_pair = -1;   // stiffness item: the mass item of the same mesh fused into its launch
+  bool fused = false;   // mass item assembled inside its stiffness item's launch
 };
 
 }  // namespace
@@ -307,16 +313,30 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     gsegs.push_back(g);
     etot += m->E;
   }
+  // stiffness and mass of the same affine mesh run as one fused pass
+  for (size_t i = 0; i < items.size(); ++i) {
+    Item& k = items[i];
+    if (!k.batched || !k.affine || k.d->op != LLFEM_OP_STIFFNESS || k.mass_pair >= 0) continue;
+    for (size_t j = 0; j < items.size(); ++j) {
+      Item& mi = items[j];
+      if (j == i || !mi.batched || !mi.affine || mi.fused || mi.d->op != LLFEM_OP_MASS || mi.m != k.m) continue;
+      k.mass_pair = (int)j;
+      mi.fused = true;
+      break;
+    }
+  }
   for (Item& it : items) {
-    if (!it.batched) continue;
+    if (!it.batched || it.fused) continue;
     llfem_mesh* m = it.m;
     const llfem_op_desc* d = it.d;
     const int op = d->op;
+    const Item* mp = it.mass_pair >= 0 ? &items[it.mass_pair] : nullptr;
+    if (mp && !m->d_valM) LLFEM_TRY(dalloc(ctx, &m->d_valM, (size_t)m->nadj * 3));
     double** outp = op == LLFEM_OP_STIFFNESS ? &m->d_valK : &m->d_valM;
     const int blk = op == LLFEM_OP_STIFFNESS ? 9 : 3;
     if (!*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)m->nadj * blk));
     const int npw = 64 / m->npe;
-    const int blkl = op == LLFEM_OP_STIFFNESS ? 9 : 1;
+    const int blkl = mp ? 10 : (op == LLFEM_OP_STIFFNESS ? 9 : 1);
     for (const PanelLaunch& L : m->launches) {
       BatchSeg s{};
       PanelParams& P = s.P;
@@ -339,13 +359,19 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
       }
       P.rho = d->rho;
       P.out = *outp;
+      if (mp) {
+        P.rho = mp->d->rho;
+        P.tab2 = mp->tab;
+        P.out2 = m->d_valM;
+      }
       P.lds_per_wave = npw * L.max_r * blkl + 9;
       max_lds = std::max<size_t>(max_lds, (size_t)P.lds_per_wave);
-      s.kind = batch_kind(npe_index(m->npe), op, it.affine);
+      s.kind = batch_kind(npe_index(m->npe), mp ? kOpKM : op, it.affine);
       bsegs.push_back(s);
       wtot += L.waves;
     }
     alg_bytes += 24.0 * m->N + 4.0 * m->npe * m->E + 8.0 * m->nadj * blk;
+    if (mp) alg_bytes += 24.0 * m->N + 4.0 * m->npe * m->E + 8.0 * m->nadj * 3;
   }
   // waves per block: 4 unless the widest panel does not fit
   const size_t wave_bytes = sizeof(double) * max_lds;

@@ -157,7 +157,14 @@ struct PanelParams {
   double* out;
   int lds_per_wave;    // doubles
   double C[81];        // general constitutive tensor C[d][al][c][be]
+  const double* tab2;  // kOpKM: the mass table beside the stiffness table
+  double* out2;        // kOpKM: the mass values
 };
+
+// Stiffness and consistent mass of one mesh in the same pass (batched affine
+// path): each step adds the K block and the mass scalar of its node pair, so
+// the records, geometry and panel round trips are shared.
+constexpr int kOpKM = 2;
 
 // Panel update: plain read-add-write.  No two lanes of a step address the
 // same live row; successive steps stay ordered because LDS executes one wave's
@@ -248,13 +255,18 @@ __device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Rea
 template <typename Real>
 struct TabBlk {
   Real v[9];
+  Real m;      // kOpKM: the mass table entry
 };
 
 // Operator-table entry of (a, b): Ghat (9 values) for stiffness, Mhat for mass.
 template <int NPE, int OP, bool AFFINE, typename Real>
-__device__ __forceinline__ TabBlk<Real> load_tab(const double* tab, const float* tabf, int a, int b) {
+__device__ __forceinline__ TabBlk<Real> load_tab(const double* tab, const float* tabf, const double* tab2, int a,
+                                                 int b) {
   TabBlk<Real> t;
-  if constexpr (AFFINE && OP == LLFEM_OP_STIFFNESS) {
+  if constexpr (OP == kOpKM) {
+    load9<Real>(tab + (size_t)(a * NPE + b) * 9, tabf + (size_t)(a * NPE + b) * 9, t.v);
+    t.m = (Real)tab2[a * NPE + b];
+  } else if constexpr (AFFINE && OP == LLFEM_OP_STIFFNESS) {
     load9<Real>(tab + (size_t)(a * NPE + b) * 9, tabf + (size_t)(a * NPE + b) * 9, t.v);
   } else if constexpr (AFFINE) {
     t.v[0] = (Real)tab[a * NPE + b];
@@ -267,8 +279,8 @@ __device__ __forceinline__ TabBlk<Real> load_tab(const double* tab, const float*
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
 __device__ __forceinline__ void contribute(const PanelParams& P, const double* geo, const double* tab,
                                            const TabBlk<Real>& tb, double* __restrict__ wl, int base, int R,
-                                           int b, int e, int a, int s, const Geo<Real>& g) {
-  if constexpr (OP == LLFEM_OP_STIFFNESS) {
+                                           int b, int e, int a, int s, const Geo<Real>& g, int mofs = 0) {
+  if constexpr (OP == LLFEM_OP_STIFFNESS || OP == kOpKM) {
     Real G[9];
     Real scale;
     if constexpr (AFFINE) {
@@ -306,6 +318,7 @@ __device__ __forceinline__ void contribute(const PanelParams& P, const double* g
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int d = 0; d < 3; ++d) dst[c * 3 * R + d] = old[c * 3 + d] + (double)K[d * 3 + c];
+    if constexpr (OP == kOpKM) lds_add(wl + base + mofs + s, P.rho * (double)g.wd * (double)tb.m);
     __builtin_amdgcn_wave_barrier();
   } else {
     double mv;
@@ -375,7 +388,7 @@ __device__ __forceinline__ StepRec load_rec(const void* recs, int rec_bytes, int
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
 __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int nw, double* wl) {
   constexpr int NPW = 64 / NPE;
-  constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : 1;
+  constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : (OP == kOpKM ? 10 : 1);
   const int lane = threadIdx.x & 63;
   double* const dummy = wl + (size_t)NPW * P.max_r * BLK;   // 9 doubles
   const int n = lane / NPE;
@@ -397,7 +410,10 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
   const double* const geo = (const double*)gp(P.geo);
   const double* const tab = (const double*)gp(P.tab);
   const float* const tabf = (const float*)gp(P.tabf);
+  const double* const tab2 = (const double*)gp(P.tab2);
   double* const out = (double*)gp(P.out);
+  double* const out2 = (double*)gp(P.out2);
+  const int mofs = 9 * P.max_r;   // kOpKM: mass panel after the stiffness panel
 
   // node descriptors of the current and next round
   const int2* desc = (const int2*)gp(P.desc) + w0 * (int64_t)P.rounds * NPW;
@@ -413,7 +429,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
 #pragma unroll
   for (int v = 0; v < PF; ++v) {
     g[v] = load_geo<Real>(geo + (size_t)rec[v].e * kGeoStride);
-    tb[v] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, rec[v].a < 0 ? 0 : rec[v].a, b);
+    tb[v] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, rec[v].a < 0 ? 0 : rec[v].a, b);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -429,7 +445,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
         contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, geo, tab, tb[u % PF], live ? pan : dummy,
                                                      0, live ? dcur.y : 0, b, cr.e,
                                                      cr.a < 0 ? 0 : cr.a, live ? cr.s : 0,
-                                                     g[u % PF]);
+                                                     g[u % PF], live ? mofs : 0);
         if (++t == cnt) {   // round complete: write the NPW panels, clear them
           t = 0;
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -439,11 +455,19 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
             const int Rm = __shfl(dcur.y, m * NPE);
             const int64_t om = (int64_t)__shfl(dcur.x, m * NPE);
             double* pm = wl + (size_t)m * P.max_r * BLK;
-            if constexpr (OP == LLFEM_OP_STIFFNESS) {
+            if constexpr (OP == LLFEM_OP_STIFFNESS || OP == kOpKM) {
               double* o = out + 9 * om;
               for (int i = lane; i < 9 * Rm; i += 64) {
                 __builtin_nontemporal_store(pm[i], o + i);
                 pm[i] = 0.0;
+              }
+              if constexpr (OP == kOpKM) {
+                double* o2 = out2 + 3 * om;
+                const double* pmm = pm + mofs;
+                for (int i = lane; i < 3 * Rm; i += 64) {
+                  const int c = i / Rm;
+                  __builtin_nontemporal_store(pmm[i - c * Rm], o2 + i);
+                }
               }
             } else {
               double* o = out + 3 * om;
@@ -453,10 +477,17 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
               }
             }
           }
-          if constexpr (OP != LLFEM_OP_STIFFNESS) {
+          if constexpr (OP == LLFEM_OP_MASS) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             for (int i = lane; i < NPW * P.max_r; i += 64) wl[i] = 0.0;
+          } else if constexpr (OP == kOpKM) {   // clear the mass panels once all three copies are out
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int i = lane; i < NPW * P.max_r; i += 64) {
+              const int m = i / P.max_r;
+              wl[(size_t)m * P.max_r * BLK + mofs + (i - m * P.max_r)] = 0.0;
+            }
           }
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           __builtin_amdgcn_wave_barrier();
@@ -468,7 +499,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
       // ring refill: geometry/table for step s+PF, record for step s+2PF
       const StepRec& nx = rec[(u + PF) % (2 * PF)];
       g[u % PF] = load_geo<Real>(geo + (size_t)nx.e * kGeoStride);
-      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, nx.a < 0 ? 0 : nx.a, b);
+      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, nx.a < 0 ? 0 : nx.a, b);
       rec[u] = load_rec<NPE, MapT>(recs, P.rec_bytes, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
     }
   }
