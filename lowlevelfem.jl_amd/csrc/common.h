@@ -152,6 +152,9 @@ struct llfem_mesh {
   int64_t* d_cptr = nullptr;      // [nadj+1]
   int32_t* d_csrc = nullptr;      // [ncontrib] staged block index e*npe^2 + a*npe + b
   int32_t* d_blk_node = nullptr;  // [nadj] column node of each nnz block
+  int32_t* d_direct = nullptr;    // [E*npe^2] CSC block a staged block is the only contribution to, else -1
+  int32_t* d_multi = nullptr;     // [nmulti] CSC blocks with more than one contribution
+  int64_t nmulti = 0;
   void* d_ho = nullptr;           // staged blocks + Z
   size_t ho_bytes = 0;
   // AMG hierarchies kept per support (key: hash of the constrained-DoF list),

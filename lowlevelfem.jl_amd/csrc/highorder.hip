@@ -101,6 +101,12 @@ struct GramParams {
   double C[81];
   int iso;
   double* S;              // staged blocks S[e][col node a][row node b][c][d]
+  // a block that is the only contribution to its CSC block goes straight to
+  // the CSC values (direct[staged index] = CSC block, -1: staged)
+  const int32_t* direct;
+  const int32_t* blk_node;
+  const int64_t* adj_ptr;
+  double* out;
 };
 
 // Upper-triangle tile pair index -> (ti, tj), ti <= tj.
@@ -166,13 +172,25 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, 
             K[d * 3 + c] = acc;
           }
       }
-      // staged block of column node cn, row node rn: [c][d] = K[d][c]
+      // block of column node cn, row node rn: [c][d] = K[d][c]
       const int cn = pass == 0 ? a : b, rn = pass == 0 ? b : a;
-      double* o = S + ((size_t)cn * npe + rn) * 9;
+      const int32_t p = P.direct[((size_t)e * npe + cn) * npe + rn];
+      if (p >= 0) {   // CSC panel of node j: value (c, slot s, d) at 9 a0 + c 3R + 3 s + d
+        const int64_t j = P.blk_node[p];
+        const int64_t a0 = P.adj_ptr[j];
+        const int64_t R = P.adj_ptr[j + 1] - a0;
+        double* o = P.out + 9 * a0 + 3 * (p - a0);
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) o[c * 3 + d] = K[d * 3 + c];
+          for (int d = 0; d < 3; ++d) o[c * 3 * R + d] = K[d * 3 + c];
+      } else {
+        double* o = S + ((size_t)cn * npe + rn) * 9;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) o[c * 3 + d] = K[d * 3 + c];
+      }
     }
   }
 }
@@ -307,12 +325,16 @@ void k_gram_bf16x3(GramParams P) {
 
 // One thread per node-pair block p: sum its staged element blocks in ascending
 // element order and write the 3x3 block into the node panel.
-__global__ void k_stage_gather(int64_t nblk, const int32_t* __restrict__ blk_node,
+// CSC blocks with more than one contribution: sum the staged blocks in
+// ascending element order (the single-contribution ones were written by the
+// Gram epilogue).
+__global__ void k_stage_gather(int64_t nmulti, const int32_t* __restrict__ multi, const int32_t* __restrict__ blk_node,
                                const int64_t* __restrict__ adj_ptr, const int64_t* __restrict__ cptr,
                                const int32_t* __restrict__ csrc, const double* __restrict__ S,
                                double* __restrict__ out) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nblk) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmulti) return;
+  const int64_t p = multi[k];
   const int64_t j = blk_node[p];
   const int64_t a0 = adj_ptr[j];
   const int64_t R = adj_ptr[j + 1] - a0;
@@ -361,6 +383,20 @@ __global__ void k_blk_node(int64_t N, const int64_t* __restrict__ adj_ptr, int32
   for (int64_t p = adj_ptr[j]; p < adj_ptr[j + 1]; ++p) bn[p] = (int32_t)j;
 }
 
+__global__ void k_direct_map(int64_t nblk, const int64_t* __restrict__ cptr, const int32_t* __restrict__ csrc,
+                             int32_t* __restrict__ direct, uint8_t* __restrict__ multi_flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nblk) return;
+  const int64_t n = cptr[p + 1] - cptr[p];
+  if (n == 1) direct[csrc[cptr[p]]] = (int32_t)p;
+  multi_flag[p] = n > 1 ? 1 : 0;
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t v, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
 __global__ void k_split_contrib(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ src,
                                 int64_t* __restrict__ ptr, int64_t nblk) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,11 +439,33 @@ int build_contrib(llfem_mesh* m) {
   LLFEM_TRY(dalloc(ctx, &m->d_blk_node, (size_t)m->nadj));
   k_split_contrib<<<(unsigned)((n + 256) / 256), 256, 0, st>>>(n, tmp, m->d_csrc, m->d_cptr, m->nadj);
   k_blk_node<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_blk_node);
+  // direct-write map of single-contribution blocks, list of the others
+  const int64_t nstaged = m->E * (int64_t)m->npe * m->npe;
+  uint8_t* flags = nullptr;
+  int64_t* d_num = nullptr;
+  LLFEM_TRY(dalloc(ctx, &m->d_direct, (size_t)nstaged));
+  LLFEM_TRY(dalloc(ctx, &flags, (size_t)m->nadj));
+  LLFEM_TRY(dalloc(ctx, &m->d_multi, (size_t)std::max<int64_t>(m->nadj, 1)));
+  LLFEM_TRY(dalloc(ctx, &d_num, 1));
+  k_fill_i32<<<(unsigned)((nstaged + 255) / 256), 256, 0, st>>>(nstaged, -1, m->d_direct);
+  k_direct_map<<<(unsigned)((m->nadj + 255) / 256), 256, 0, st>>>(m->nadj, m->d_cptr, m->d_csrc, m->d_direct, flags);
+  hipcub::CountingInputIterator<int32_t> ids(0);
+  size_t sb = 0;
+  LLFEM_HIP(hipcub::DeviceSelect::Flagged(nullptr, sb, ids, flags, m->d_multi, d_num, (int)m->nadj, st));
+  void* d_sel = nullptr;
+  LLFEM_TRY(dmalloc(ctx, &d_sel, sb));
+  LLFEM_HIP(hipcub::DeviceSelect::Flagged(d_sel, sb, ids, flags, m->d_multi, d_num, (int)m->nadj, st));
+  int64_t nm = 0;
+  LLFEM_HIP(hipMemcpyAsync(&nm, d_num, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   LLFEM_HIP(hipGetLastError());
   LLFEM_HIP(hipStreamSynchronize(st));
+  dfree(ctx, d_sel, sb);
+  drelease(ctx, flags, (size_t)m->nadj);
+  drelease(ctx, d_num, 1);
   drelease(ctx, node, (size_t)m->ninc);
   drelease(ctx, keys, (size_t)n);
   drelease(ctx, tmp, (size_t)n);
+  m->nmulti = nm;
   m->ncontrib = n;
   return LLFEM_OK;
 }
@@ -465,6 +523,10 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.iso = C81 == nullptr;
   if (C81) std::memcpy(P.C, C81, sizeof(P.C));
   P.S = S;
+  P.direct = m->d_direct;
+  P.blk_node = m->d_blk_node;
+  P.adj_ptr = m->d_adj_ptr;
+  P.out = out;
   const size_t lds = sizeof(double) * 48 * kHoTile;
   auto fn = bf ? k_gram_bf16x3 : k_gram_f64;
   LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -474,8 +536,9 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
   hipLaunchKernelGGL(fn, grid, dim3(256), lds, st, P);
   LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
-  k_stage_gather<<<(unsigned)((m->nadj + 255) / 256), 256, 0, st>>>(m->nadj, m->d_blk_node, m->d_adj_ptr,
-                                                                    m->d_cptr, m->d_csrc, S, out);
+  if (m->nmulti > 0)
+    k_stage_gather<<<(unsigned)((m->nmulti + 255) / 256), 256, 0, st>>>(m->nmulti, m->d_multi, m->d_blk_node,
+                                                                         m->d_adj_ptr, m->d_cptr, m->d_csrc, S, out);
   LLFEM_HIP(hipGetLastError());
   return LLFEM_OK;
 }

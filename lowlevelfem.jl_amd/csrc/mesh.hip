@@ -435,6 +435,8 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_cptr, m->nadj + 1);
   drelease(c, m->d_csrc, m->ncontrib);
   drelease(c, m->d_blk_node, m->nadj);
+  drelease(c, m->d_direct, (size_t)m->E * m->npe * m->npe);
+  drelease(c, m->d_multi, (size_t)std::max<int64_t>(m->nmulti, 1));
   if (m->d_ho) dfree(c, m->d_ho, m->ho_bytes);
   for (auto& kv : m->amg) {
     amg_free(c, kv.second.first);
