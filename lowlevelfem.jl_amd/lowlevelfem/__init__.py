@@ -18,8 +18,9 @@ hot path (LowLevelFEM.jl 1.13.31):
     constrainedDoFs / freeDoFs / allDoFs / nodesOnPhysicalGroup
                          src/general.jl:5249-5492
     build_csc_pattern    src/multifield.jl:2083-2250
-    SymGrad / integral   the ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); ...) DSL path,
-                         src/multifield.jl:5295-5339, assemble_operator :2630-3280
+    SymGrad / Id / integral
+                         the ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); ...) and ∫(Id(Pu) ⋅ c ⋅ Id(Pu); ...)
+                         DSL paths, src/multifield.jl:5295-5391, assemble_operator :2630-3280
 
 Differences that follow from the host language: DoF and node indices are
 0-based (dof = 3*node + c); sparse matrices are scipy.sparse.csc_matrix with
@@ -43,7 +44,7 @@ __all__ = [
     "Material", "Problem", "SystemMatrix", "VectorField", "BoundaryCondition", "LoadCondition",
     "structured_box_mesh", "stiffnessMatrix", "massMatrix", "loadVector", "solveDisplacement",
     "displacementConstraint", "load", "constrainedDoFs", "freeDoFs", "allDoFs",
-    "nodesOnPhysicalGroup", "build_csc_pattern", "SymGrad", "integral", "dropzeros",
+    "nodesOnPhysicalGroup", "build_csc_pattern", "SymGrad", "Id", "integral", "dropzeros",
     "elastic_constants", "Mesh",
 ]
 
@@ -270,7 +271,16 @@ def stiffnessMatrix(problem: Problem, elements=(), forceOneThread=True, gauss=No
     A = _csc(problem.ndofs(), colptr, rowval, val)
     if dropzeros_:
         A = dropzeros(A)
-    return SystemMatrix(A, problem, st.as_dict())
+    return SystemMatrix(A, problem, _device_k(problem, st))
+
+
+def _device_k(problem: Problem, st) -> dict:
+    """Stats of a stiffness assembly, tagged with the device copy it left on
+    the problem's mesh (solveDisplacement's preconditioner="amg" solves that copy)."""
+    problem._k_version = getattr(problem, "_k_version", 0) + 1
+    d = st.as_dict()
+    d["device_k_version"] = problem._k_version
+    return d
 
 
 class SymGrad:
@@ -281,11 +291,53 @@ class SymGrad:
         self.P = P
 
 
-def integral(a: SymGrad, D, b: SymGrad, gauss=None, csc_matrix=None, precision="fp64"):
+class Id:
+    """Id(P): identity operator applied to a vector field (src/multifield.jl:4463-4475,
+    IdOp :144; build_B! :500)."""
+
+    def __init__(self, P: Problem):
+        self.P = P
+
+
+def _integral_id(a, c, b, gauss, csc_matrix):
+    """∫(Id(Pu) ⋅ c ⋅ Id(Pu)) with a scalar c: the consistent-mass shape
+    ρ Σ_q N_a N_b |J| w δ_ij (src/multifield.jl:5295-5391 with IdOp).  Without
+    csc_matrix: dropzeros! of the block-diagonal entries; with it the
+    structural pattern is kept and the off-diagonal entries of every 3x3
+    block are stored zeros, as the reference's CSC scatter leaves them."""
+    if not (isinstance(a, Id) and isinstance(b, Id) and a.P is b.P):
+        raise NotImplementedError("only ∫(Id(Pu) ⋅ c ⋅ Id(Pu)) with one field is on the GPU path")
+    if np.size(c) != 1:
+        raise NotImplementedError("∫(Id ⋅ C ⋅ Id): only a scalar coefficient is on the GPU path")
+    rho = float(np.asarray(c, dtype=np.float64).reshape(-1)[0])
+    P = a.P
+    n = P.ndofs()
+    dev = P.device()
+    val, st = dev.assemble(_lib.OP_MASS, _rule(P, gauss), rho=rho, lumped=False)
+    if csc_matrix is None:
+        mcp, mrv = dev.pattern(_lib.OP_MASS)
+        return SystemMatrix(dropzeros(_csc(n, mcp, mrv, val)), P, st.as_dict())
+    K = csc_matrix.A if isinstance(csc_matrix, SystemMatrix) else csc_matrix
+    cp, rv = dev.pattern(_lib.OP_STIFFNESS)
+    if K.shape != (n, n) or not (np.array_equal(K.indptr, cp) and np.array_equal(K.indices, rv)):
+        raise ValueError("assemble_operator: csc_matrix has an incompatible size/pattern")
+    # the mass values sit where the row component equals the column component;
+    # per column those entries are in the same order as the mass pattern's rows
+    col = np.repeat(np.arange(n, dtype=np.int64), np.diff(cp))
+    diag = (rv % 3) == (col % 3)
+    K.data[:] = 0.0
+    K.data[diag] = val
+    return SystemMatrix(K, P, st.as_dict())
+
+
+def integral(a, D, b, gauss=None, csc_matrix=None, precision="fp64"):
     """∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); gauss, csc_matrix) (src/multifield.jl:5295-5339).
     D is the 6x6 coefficient in SymGrad row order [xx,yy,zz,xy,xz,yz].
     Without csc_matrix the result gets dropzeros! (:5336); with it the
-    structural pattern is kept and the caller's matrix is filled in place."""
+    structural pattern is kept and the caller's matrix is filled in place.
+    integral(Id(Pu), c, Id(Pu), ...) is the mass-shaped form (scalar c)."""
+    if isinstance(a, Id) or isinstance(b, Id):
+        return _integral_id(a, D, b, gauss, csc_matrix)
     if not isinstance(a, SymGrad) or not isinstance(b, SymGrad) or a.P is not b.P:
         raise NotImplementedError("only ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu)) is on the GPU path")
     P = a.P
@@ -298,9 +350,9 @@ def integral(a: SymGrad, D, b: SymGrad, gauss=None, csc_matrix=None, precision="
         if K.shape != (P.ndofs(), P.ndofs()) or K.nnz != val.size:
             raise ValueError("assemble_operator: csc_matrix has an incompatible size/pattern")
         K.data[:] = val
-        return SystemMatrix(K, P, st.as_dict())
+        return SystemMatrix(K, P, _device_k(P, st))
     colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
-    return SystemMatrix(dropzeros(_csc(P.ndofs(), colptr, rowval, val)), P, st.as_dict())
+    return SystemMatrix(dropzeros(_csc(P.ndofs(), colptr, rowval, val)), P, _device_k(P, st))
 
 
 def massMatrix(problem: Problem, lumped=True, gauss=None) -> SystemMatrix | None:
@@ -354,7 +406,11 @@ def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=Fal
 
     The GPU path is Jacobi-PCG (method "fp64", or "mixed" = FP32 inner CG
     with FP64 refinement).  iterative=false (the reference's direct solve)
-    runs the same PCG to reltol = 1e-12; iterative=true honours reltol."""
+    runs the same PCG to reltol = 1e-12; iterative=true honours reltol.
+    preconditioner (the reference passes it to cg as Pl, :3736): "jacobi", or
+    "amg" — mixed-precision CG with the aggregation-AMG V-cycle on the
+    stiffness kept on the problem's device mesh; K must be the result of the
+    last stiffnessMatrix / integral(SymGrad...) call on that problem."""
     problem = K.model
     if f.type != "v3D":
         raise ValueError(f"solveDisplacement: wrong type of 'f': ({f.type})")
@@ -374,6 +430,19 @@ def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=Fal
     maxiter = maxiter or max(problem.non * problem.dim, 1000)
     meth = _lib.PCG_MIXED_REFINE if method == "mixed" else _lib.PCG_JACOBI_FP64
     dev = problem.device()
+    if preconditioner == "amg":
+        if K.stats.get("device_k_version") is None or \
+                K.stats["device_k_version"] != getattr(problem, "_k_version", None):
+            raise ValueError("solveDisplacement: preconditioner=\"amg\" solves the stiffness kept on the "
+                             "device; K is not the last one assembled for this problem")
+        fa = np.ascontiguousarray(f.a[:, 0], dtype=np.float64)
+        u, st = dev.solve(fa, fixed, ufix, method=_lib.PCG_AMG_RELIABLE, reltol=tol, maxiter=maxiter)
+        out = f.copy()
+        out.a[:, 0] = u
+        out.stats = st.as_dict()
+        return out
+    if preconditioner not in ("jacobi", None):
+        raise NotImplementedError(f"solveDisplacement: preconditioner {preconditioner!r} is not on the GPU path")
     A = K.A
     fa = np.ascontiguousarray(f.a[:, 0], dtype=np.float64)
     u, st = _lib.solve_csc(dev.ctx, A.indptr, A.indices, A.data, fa, fixed, ufix, method=meth,
