@@ -545,8 +545,8 @@ def main():
                          "(reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
     out = WORKLOADS[args.workload](args)
     print(json.dumps(out), flush=True)
