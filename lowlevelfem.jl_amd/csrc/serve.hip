@@ -51,6 +51,12 @@ bool trace_on() {
 
 double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
 
+// trace time stamp: ms since the first traced event of the process
+double trace_t() {
+  static const Clock::time_point t0 = Clock::now();
+  return ms_since(t0);
+}
+
 __global__ void k_load_from_weights(int64_t N, const double* __restrict__ s, double tx, double ty, double tz,
                                     double* __restrict__ f) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,6 +124,7 @@ struct Req {
   SysSpec spec;              // the system (kept while it waits for its AMG hierarchy)
   uint64_t support_key = 0;
   bool waiting = false;      // admitted, its AMG hierarchy still building
+  int64_t cm_nodes = 0;      // mesh nodes (trace)
   int64_t iterations = 0, updates = 0;
   double relres = 0.0;
 };
@@ -222,6 +229,7 @@ int admit(llfem_server* S, Req* q) {
   CachedMesh* cm = &it->second;
   t_mesh = ms_since(t0);
   q->cm = cm;
+  q->cm_nodes = cm->m->N;
   cm->users++;
   cm->last = Clock::now();
   llfem_mesh* m = cm->m;
@@ -337,16 +345,16 @@ int admit(llfem_server* S, Req* q) {
         q->waiting = true;
         q->t_admit = Clock::now();
         if (trace_on())
-          std::fprintf(stderr, "[serve] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f -> waits\n",
-                       (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg - t_k);
+          std::fprintf(stderr, "[serve %9.1f] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f -> waits\n",
+                       trace_t(), (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg - t_k);
         return LLFEM_OK;
       }
     }
     q->spec = sp;
     if ((rc = q->eng->add(sp, &q->sys))) break;
     if (trace_on())
-      std::fprintf(stderr, "[serve] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f add %.1f ms\n",
-                   (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg > 0 ? t_amg - t_k : 0.0,
+      std::fprintf(stderr, "[serve %9.1f] admit #%lld N=%lld mesh %.1f wm %.1f K %.1f amg %.1f add %.1f ms\n",
+                   trace_t(), (long long)q->id, (long long)m->N, t_mesh, t_wm - t_mesh, t_k - t_wm, t_amg > 0 ? t_amg - t_k : 0.0,
                    ms_since(t0) - (t_amg > 0 ? t_amg : t_k));
   } while (0);
   if (rc != LLFEM_OK) {
@@ -386,6 +394,9 @@ void complete(llfem_server* S, Req* q) {
   free_request_buffers(ctx, q);
   release_mesh_use(q);
   q->t_done = Clock::now();
+  if (trace_on())
+    std::fprintf(stderr, "[serve %9.1f] done #%lld N=%lld %lld it, %lld updates\n", trace_t(), (long long)q->id,
+                 (long long)q->cm_nodes, (long long)q->iterations, (long long)q->updates);
   std::lock_guard<std::mutex> lk(S->mu);
   q->rc = rc;
   q->done = true;
@@ -464,7 +475,8 @@ void worker_loop(llfem_server* S) {
         const Clock::time_point ta = Clock::now();
         const int rc = q->eng->add(q->spec, &q->sys);
         if (trace_on())
-          std::fprintf(stderr, "[serve] start #%lld after hierarchy: add %.1f ms\n", (long long)q->id, ms_since(ta));
+          std::fprintf(stderr, "[serve %9.1f] start #%lld after hierarchy: add %.1f ms\n", trace_t(), (long long)q->id,
+                       ms_since(ta));
         if (rc == LLFEM_ENOMEM && (S->eng32->any_running() || S->eng64->any_running())) {
           waiting = true;          // retry once running systems release memory
           continue;
@@ -498,7 +510,8 @@ void worker_loop(llfem_server* S) {
     for (BatchSolver* e : {S->eng32.get(), S->eng64.get()})
       if (rc == LLFEM_OK && e->any_running()) rc = e->run(S->chunk, &fin);
     if (trace_on())
-      std::fprintf(stderr, "[serve] chunk %.1f ms, %zu active, %zu finished\n", ms_since(tc), S->active.size(),
+      std::fprintf(stderr, "[serve %9.1f] chunk %.1f ms, %zu active, %zu finished\n", trace_t(), ms_since(tc),
+                   S->active.size(),
                    fin.size());
     std::vector<std::shared_ptr<Req>> done;
     {
