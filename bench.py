@@ -100,6 +100,33 @@ def calib_copy(nbytes):
     hip.hipFree(b)
 
 
+def fill_rate(nbytes, reps=10):
+    """GB/s of HIP's own device fill (hipMemsetAsync) over nbytes, timed with
+    HIP events: the streaming-write rate this GPU reaches on a buffer the size
+    of the CSC values, printed beside the roofline as the practical write floor."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) != 0:
+        return None
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    for _ in range(2):
+        hip.hipMemsetAsync(a, 0, ctypes.c_size_t(nbytes), None)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemsetAsync(a, 0, ctypes.c_size_t(nbytes), None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    return nbytes / (ms.value / reps * 1e-3) / 1e9
+
+
 def bench_tet10(args):
     from lowlevelfem import _lib, mesh as M, refelem as R
     cells = tuple(int(x) for x in args.cells.split(",")) if args.cells else CELLS
@@ -163,6 +190,11 @@ def bench_tet10(args):
                                     "elem_per_s": m.E / (st32.kernel_ms * 1e-3)},
         "setup_s": setup_s,
     }
+    fr = fill_rate(8 * nnz)
+    if fr:
+        # the nzval write alone at the fill rate: a floor for any assembly
+        out["write_floor"] = {"fill_gbs": fr, "nzval_write_ms": 8.0 * nnz / (fr * 1e9) * 1e3,
+                              "achieved_over_fill": achieved / fr}
     if args.calib_copy:
         calib_copy(args.calib_copy)
     if not args.no_cpu:

@@ -26,6 +26,7 @@
 //     Gauss point from per-GP inverse Jacobians.
 //   * K_ba[d][c] = lam G[d][c] + mu G[c][d] + mu tr(G) delta_dc for Hooke's
 //     law, or sum C[d][al][c][be] G[al][be] for a general 6x6 D.
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -55,6 +56,72 @@ PanelFn pick(int op, bool affine, bool iso, bool fp32) {
   }
   if (iso) return fp32 ? k_panel<NPE, MapT, 0, false, true, float> : k_panel<NPE, MapT, 0, false, true, double>;
   return fp32 ? k_panel<NPE, MapT, 0, false, false, float> : k_panel<NPE, MapT, 0, false, false, double>;
+}
+
+// Stiffness, affine, Hooke's law, one-byte maps (tet4, hex8, tet10): the
+// variant whose panels leave in a compile-time number of stores per round.
+template <int NPE, typename Real>
+PanelFn pick_fixed_stores(int ks) {
+  switch (ks) {
+    case 2: return k_panel<NPE, uint8_t, 0, true, true, Real, 2>;
+    case 3: return k_panel<NPE, uint8_t, 0, true, true, Real, 3>;
+    case 4: return k_panel<NPE, uint8_t, 0, true, true, Real, 4>;
+    case 5: return k_panel<NPE, uint8_t, 0, true, true, Real, 5>;
+    case 6: return k_panel<NPE, uint8_t, 0, true, true, Real, 6>;
+    case 8: return k_panel<NPE, uint8_t, 0, true, true, Real, 8>;
+    case 10: return k_panel<NPE, uint8_t, 0, true, true, Real, 10>;
+    case 12: return k_panel<NPE, uint8_t, 0, true, true, Real, 12>;
+    case 16: return k_panel<NPE, uint8_t, 0, true, true, Real, 16>;
+    default: return nullptr;
+  }
+}
+
+// Smallest instantiated store count covering a panel of 9 * max_r doubles
+// (0: none, use the run-time store loop).
+int fixed_store_count(int max_r) {
+  const int need = (9 * max_r + 63) / 64;
+  for (int ks : {2, 3, 4, 5, 6, 8, 10, 12, 16})
+    if (ks >= need) return ks;
+  return 0;
+}
+
+PanelFn select_fixed_stores(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32, int max_r) {
+  if (map_bytes != 1 || op != LLFEM_OP_STIFFNESS || !affine || !iso) return nullptr;
+  const int ks = fixed_store_count(max_r);
+  if (ks == 0) return nullptr;
+  switch (npe) {
+    case 4: return fp32 ? pick_fixed_stores<4, float>(ks) : pick_fixed_stores<4, double>(ks);
+    case 8: return fp32 ? pick_fixed_stores<8, float>(ks) : pick_fixed_stores<8, double>(ks);
+    case 10: return fp32 ? pick_fixed_stores<10, float>(ks) : pick_fixed_stores<10, double>(ks);
+    default: return nullptr;
+  }
+}
+
+// Round-per-iteration variants (k_panel_rounds) for the node classes of
+// structured tet10 and hex8 meshes: (npe, steps per node, ring, stores per
+// panel).  A launch whose (npe, cnt) is listed and whose panels fit the
+// store count runs it; every other launch keeps k_panel.
+template <typename Real>
+PanelFn select_rounds_real(int npe, int cnt, int ks_needed) {
+  struct V {
+    int npe, cnt, ks;
+    PanelFn fn;
+  };
+  static const V table[] = {
+      {10, 4, 4, k_panel_rounds<10, Real, 4, 2, 4>},   {10, 4, 6, k_panel_rounds<10, Real, 4, 2, 6>},
+      {10, 6, 4, k_panel_rounds<10, Real, 6, 3, 4>},   {10, 6, 6, k_panel_rounds<10, Real, 6, 3, 6>},
+      {10, 24, 10, k_panel_rounds<10, Real, 24, 3, 10>}, {10, 24, 12, k_panel_rounds<10, Real, 24, 3, 12>},
+      {8, 8, 4, k_panel_rounds<8, Real, 8, 2, 4>},     {8, 8, 6, k_panel_rounds<8, Real, 8, 2, 6>},
+  };
+  for (const V& v : table)
+    if (v.npe == npe && v.cnt == cnt && v.ks >= ks_needed) return v.fn;
+  return nullptr;
+}
+
+PanelFn select_rounds(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32, int cnt, int max_r) {
+  if (map_bytes != 1 || op != LLFEM_OP_STIFFNESS || !affine || !iso) return nullptr;
+  const int need = (9 * max_r + 7 + 63) / 64;   // up to 7 doubles of window alignment
+  return fp32 ? select_rounds_real<float>(npe, cnt, need) : select_rounds_real<double>(npe, cnt, need);
 }
 
 PanelFn select_panel(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32) {
@@ -272,6 +339,8 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
             P.C[((dd * 3 + al) * 3 + c) * 3 + be] = d->D[voigt(dd, al) * 6 + voigt(c, be)];
   P.rho = d->rho;
   P.out = target;
+  if (!ctx->d_sink) LLFEM_TRY(dalloc(ctx, &ctx->d_sink, 64));
+  P.sink = ctx->d_sink;
   const int npw = 64 / npe;
   const int blkl = (op == LLFEM_OP_STIFFNESS) ? 9 : 1;
   if (highorder)
@@ -307,10 +376,16 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     const int64_t used = (L.waves + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
     const size_t lds_bytes = wave_bytes * wpb;
+    PanelFn lfn = fn;
+    if (!std::getenv("LLFEM_PANEL_RUNTIME_STORES")) {   // A/B knob: the run-time store loop
+      if (PanelFn f = select_fixed_stores(npe, m->map_bytes, op, affine, iso, fp32, L.max_r)) lfn = f;
+      if (!std::getenv("LLFEM_PANEL_NO_ROUNDS"))        // A/B knob: the step-unrolled kernel
+        if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) lfn = f;
+    }
     if (lds_bytes > 65536)
-      LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+      LLFEM_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds_bytes));
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * wpb), lds_bytes, st, P);
+    hipLaunchKernelGGL(lfn, dim3(blocks), dim3(64 * wpb), lds_bytes, st, P);
   }
   LLFEM_HIP(hipGetLastError());
   if (!highorder) LLFEM_HIP(hipEventRecord(ctx->ev[2], st));

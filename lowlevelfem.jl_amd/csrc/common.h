@@ -98,6 +98,7 @@ struct llfem_ctx {
   std::map<uint64_t, llfem::DevTable> tables;      // by content hash
   void* d_scratch = nullptr;                       // batch descriptors + queue counter
   size_t scratch_bytes = 0;
+  double* d_sink = nullptr;                        // 64 doubles: target of padding-lane panel stores
 };
 
 struct llfem_mesh {

@@ -103,6 +103,7 @@ int llfem_ctx_destroy(llfem_ctx* ctx) {
   for (auto& kv : ctx->tables) llfem::dfree(ctx, kv.second.ptr, kv.second.bytes);
   ctx->tables.clear();
   if (ctx->d_scratch) llfem::dfree(ctx, ctx->d_scratch, ctx->scratch_bytes);
+  if (ctx->d_sink) llfem::dfree(ctx, ctx->d_sink, 64 * sizeof(double));
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LLFEM_OK;

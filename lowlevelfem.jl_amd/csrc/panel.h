@@ -159,6 +159,7 @@ struct PanelParams {
   double C[81];        // general constitutive tensor C[d][al][c][be]
   const double* tab2;  // kOpKM: the mass table beside the stiffness table
   double* out2;        // kOpKM: the mass values
+  double* sink;        // 64 doubles: where padding lane groups send their panel stores
 };
 
 // Stiffness and consistent mass of one mesh in the same pass (batched affine
@@ -381,12 +382,43 @@ __device__ __forceinline__ StepRec load_rec(const void* recs, int rec_bytes, int
   return o;
 }
 
+// Compact 16-byte step records (NPE <= 11, one-byte maps), kept raw in the
+// prefetch ring and decoded where they are used, so a record load is waited
+// for only when its step comes.
+typedef unsigned RawRec __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ RawRec load_raw(const void* recs, int64_t r) {
+  return __builtin_nontemporal_load(reinterpret_cast<const RawRec*>(recs) + r);
+}
+
+__device__ __forceinline__ int raw_elem(const RawRec& v) { return (int)v.x; }
+
+__device__ __forceinline__ int raw_loc(const RawRec& v) {
+  const uint32_t a8 = v.y & 0xffu;
+  return a8 == 0xffu ? -1 : (int)a8;
+}
+
+__device__ __forceinline__ int raw_slot(const RawRec& v, int b) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  const int byte = 5 + b;
+  return (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
+}
+
 // Waves [w0, w0 + nw) of a panel launch run by one wavefront, LDS panel space
 // wl (P.lds_per_wave doubles).  Consecutive waves' records and node
 // descriptors are contiguous, so a span is one longer stream of nw * rounds
 // rounds.  Uses no block-level barrier.
-template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
+//
+// KS > 0 (stiffness): every panel leaves in exactly KS 64-lane stores
+// (KS * 64 >= 9 * max_r; lanes past the panel repeat its last value, padding
+// groups write to P.sink), and the next descriptors are fetched before the
+// stores, so the descriptor wait no longer drains the store burst (gfx9
+// counts stores and loads in one in-order counter).  The round end is still
+// conditional here, so the first wait after it covers the stores; the
+// step-unrolled panel_wave_rounds below removes that too.
+template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real, int KS = 0>
 __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int nw, double* wl) {
+  static_assert(KS == 0 || OP == LLFEM_OP_STIFFNESS, "fixed store counts are for stiffness panels");
   constexpr int NPW = 64 / NPE;
   constexpr int BLK = (OP == LLFEM_OP_STIFFNESS) ? 9 : (OP == kOpKM ? 10 : 1);
   const int lane = threadIdx.x & 63;
@@ -448,8 +480,32 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
                                                      g[u % PF], live ? mofs : 0);
         if (++t == cnt) {   // round complete: write the NPW panels, clear them
           t = 0;
+          int2 dfut = make_int2(0, 0);
+          if constexpr (KS > 0) dfut = desc[(int64_t)min(round + 2, G - 1) * NPW + lane_rec];
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           __builtin_amdgcn_wave_barrier();
+          if constexpr (KS > 0) {
+            double* const sink = (double*)gp(P.sink);
+#pragma unroll
+            for (int m = 0; m < NPW; ++m) {
+              const int Rm = __builtin_amdgcn_readlane(dcur.y, m * NPE);
+              const int om = __builtin_amdgcn_readlane(dcur.x, m * NPE);
+              double* pm = wl + (size_t)m * P.max_r * BLK;
+              const bool real = Rm > 0;
+              const int last = 9 * Rm - 1;
+              double* const o = real ? out + 9 * (int64_t)om : sink;
+              double v[KS];
+              // all reads before any clear: lanes past the end re-read the last value
+#pragma unroll
+              for (int k = 0; k < KS; ++k) v[k] = pm[real ? min(lane + 64 * k, last) : 0];
+#pragma unroll
+              for (int k = 0; k < KS; ++k)
+                __builtin_nontemporal_store(v[k], o + (real ? min(lane + 64 * k, last) : lane));
+#pragma unroll
+              for (int k = 0; k < KS; ++k)
+                if (lane + 64 * k <= last) pm[lane + 64 * k] = 0.0;
+            }
+          } else {
 #pragma unroll
           for (int m = 0; m < NPW; ++m) {
             const int Rm = __shfl(dcur.y, m * NPE);
@@ -477,6 +533,7 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
               }
             }
           }
+          }
           if constexpr (OP == LLFEM_OP_MASS) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -493,7 +550,10 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
           __builtin_amdgcn_wave_barrier();
           ++round;
           dcur = dnext;
-          dnext = desc[(int64_t)min(round + 1, G - 1) * NPW + lane_rec];
+          if constexpr (KS > 0)
+            dnext = dfut;
+          else
+            dnext = desc[(int64_t)min(round + 1, G - 1) * NPW + lane_rec];
         }
       }
       // ring refill: geometry/table for step s+PF, record for step s+2PF
@@ -505,7 +565,126 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
   }
 }
 
-template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
+// One round per loop iteration: affine Hooke stiffness for a launch whose
+// nodes all take CNT steps (CNT a multiple of the ring length 2 * PFX).  The
+// CNT steps are unrolled and the round end -- descriptor fetch, KS stores per
+// panel, clears -- is straight-line code at the end of the body, so every
+// iteration issues the same vector-memory sequence and the compiler's wait
+// counts are exact everywhere: after the stores, the first PFX steps consume
+// loads issued before them without waiting for the stores, and the stores
+// have PFX steps to complete before the next wait that covers them.
+template <int NPE, typename Real, int CNT, int PFX, int KS>
+__device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t w0, double* wl) {
+  static_assert(CNT % (2 * PFX) == 0, "the ring must wrap at round ends");
+  constexpr int NPW = 64 / NPE;
+  constexpr int OP = LLFEM_OP_STIFFNESS;
+  constexpr int BLK = 9;
+  const int lane = threadIdx.x & 63;
+  double* const dummy = wl + (size_t)NPW * P.max_r * BLK;   // 9 doubles
+  const int n = lane / NPE;
+  const int b = lane - n * NPE;
+  const bool grp = n < NPW;
+  const int G = P.rounds;
+  const int64_t rbase = w0 * G * (int64_t)CNT * NPW;
+  const int64_t rlast = rbase + (int64_t)G * CNT * NPW - 1;
+  const int64_t lane_rec = grp ? n : 0;
+  double* const pan = wl + (size_t)(grp ? n : 0) * P.max_r * BLK;
+
+  for (int i = lane; i < NPW * P.max_r * BLK + 9; i += 64) wl[i] = 0.0;
+
+  const void* const recs = (const void*)gp(P.rec);
+  const double* const geo = (const double*)gp(P.geo);
+  const double* const tab = (const double*)gp(P.tab);
+  const float* const tabf = (const float*)gp(P.tabf);
+  double* const out = (double*)gp(P.out);
+  double* const sink = (double*)gp(P.sink);
+
+  const int2* desc = (const int2*)gp(P.desc) + w0 * (int64_t)G * NPW;
+  int2 dcur = desc[lane_rec];
+  int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
+
+  RawRec rec[2 * PFX];
+  Geo<Real> g[PFX];
+  TabBlk<Real> tb[PFX];
+#pragma unroll
+  for (int v = 0; v < 2 * PFX; ++v) rec[v] = load_raw(recs, min(rbase + (int64_t)v * NPW + lane_rec, rlast));
+#pragma unroll
+  for (int v = 0; v < PFX; ++v) {
+    const int a = raw_loc(rec[v]);
+    g[v] = load_geo<Real>(geo + (size_t)raw_elem(rec[v]) * kGeoStride);
+    tb[v] = load_tab<NPE, OP, true, Real>(tab, tabf, nullptr, a < 0 ? 0 : a, b);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  for (int r = 0; r < G; ++r) {
+    const bool node_live = grp && dcur.y > 0;
+#pragma unroll
+    for (int u = 0; u < CNT; ++u) {
+      const int64_t s = (int64_t)r * CNT + u;
+      const RawRec& cr = rec[u % (2 * PFX)];
+      const int ca = raw_loc(cr);
+      const bool live = node_live && ca >= 0;
+      contribute<NPE, uint8_t, OP, true, true, Real>(P, geo, tab, tb[u % PFX], live ? pan : dummy, 0,
+                                                     live ? dcur.y : 0, b, raw_elem(cr), ca < 0 ? 0 : ca,
+                                                     live ? raw_slot(cr, b) : 0, g[u % PFX], 0);
+      // ring refill: geometry/table for step s+PFX, record for step s+2PFX
+      const RawRec& nx = rec[(u + PFX) % (2 * PFX)];
+      const int na = raw_loc(nx);
+      g[u % PFX] = load_geo<Real>(geo + (size_t)raw_elem(nx) * kGeoStride);
+      tb[u % PFX] = load_tab<NPE, OP, true, Real>(tab, tabf, nullptr, na < 0 ? 0 : na, b);
+      rec[u % (2 * PFX)] = load_raw(recs, min(rbase + (s + 2 * PFX) * NPW + lane_rec, rlast));
+      // keep each step's loads in their step: the scheduler would otherwise
+      // sink them towards their uses and shorten the prefetch distance
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // round end: the descriptors of round r+2 first, then KS stores per panel
+    const int2 dfut = desc[(int64_t)min(r + 2, G - 1) * NPW + lane_rec];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = 0; m < NPW; ++m) {
+      const int Rm = __builtin_amdgcn_readlane(dcur.y, m * NPE);
+      const int om = __builtin_amdgcn_readlane(dcur.x, m * NPE);
+      double* pm = wl + (size_t)m * P.max_r * BLK;
+      const bool real = Rm > 0;
+      const int last = 9 * Rm - 1;
+      double* const o = real ? out + 9 * (int64_t)om : sink;
+      // each store covers one 512-byte-aligned window, so 64-byte sectors
+      // are split between two stores only at the ends of a panel
+      const int shift = real ? (int)(((uintptr_t)o >> 3) & 7) : 0;
+      double v[KS];
+#pragma unroll
+      for (int k = 0; k < KS; ++k) v[k] = pm[real ? min(max(lane + 64 * k - shift, 0), last) : 0];
+#pragma unroll
+      for (int k = 0; k < KS; ++k)
+        __builtin_nontemporal_store(v[k], o + (real ? min(max(lane + 64 * k - shift, 0), last) : lane));
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        const int i = lane + 64 * k - shift;
+        if (i >= 0 && i <= last) pm[i] = 0.0;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    dcur = dnext;
+    dnext = dfut;
+  }
+}
+
+template <int NPE, typename Real, int CNT, int PFX, int KS>
+__global__ __launch_bounds__(512) void k_panel_rounds(PanelParams P) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x >> 6;
+  // XCD-aware work order, as in k_panel
+  const int64_t per = gridDim.x / kXcds;
+  const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  const int64_t w = lb * (blockDim.x >> 6) + wave;
+  if (w >= P.waves) return;
+  panel_wave_rounds<NPE, Real, CNT, PFX, KS>(P, w, lds + (size_t)wave * P.lds_per_wave);
+}
+
+template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real, int KS = 0>
 __global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6;
@@ -519,7 +698,7 @@ __global__ __launch_bounds__(ISO ? 512 : 256) void k_panel(PanelParams P) {
   const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;   // logical block
   const int64_t w = lb * (blockDim.x >> 6) + wave;
   if (w >= P.waves) return;                       // no block-level barriers below
-  panel_wave<NPE, MapT, OP, AFFINE, ISO, Real>(P, w, 1, lds + (size_t)wave * P.lds_per_wave);
+  panel_wave<NPE, MapT, OP, AFFINE, ISO, Real, KS>(P, w, 1, lds + (size_t)wave * P.lds_per_wave);
 }
 
 // lumped mass: row sums of the consistent mass panels (src/linear.jl:1999-2001)

@@ -67,6 +67,27 @@ def test_stiffness_parity(gpu_ctx, label, kw, deg):
     assert np.array_equal(val, val2)
 
 
+@pytest.mark.parametrize("label,kw", [("tet10", dict(lz=2, cells=(3, 3, 8), elem="tet", order=2)),
+                                      ("tet10-classes", dict(lz=2, cells=(12, 12, 24), elem="tet", order=2)),
+                                      ("hex8", dict(n=6)), ("hex8-classes", dict(n=24)),
+                                      ("tet4", dict(n=5, elem="tet", order=1))])
+def test_fixed_store_count_path_is_bitwise_the_loop_path(gpu_ctx, monkeypatch, label, kw):
+    """Affine Hooke stiffness panels leave either in a compile-time number of
+    stores per round -- in the step-unrolled round kernels for node classes
+    with 4, 6, 8 or 24 incidences, otherwise in the generic kernel -- or
+    through the run-time store loop; all sum the same contributions in the
+    same order (the kernels may contract multiply-adds differently, so the
+    bar is 1e-14 of the largest entry, as for batched against alone)."""
+    say(f"[stiffness] fixed vs run-time panel stores {label}")
+    m = M.structured_box_mesh(**kw)
+    tab = R.rule_tables(m.etype, 2 if m.npe == 10 else 3)
+    dev = _dev(gpu_ctx, m)
+    fixed, _ = dev.assemble(_lib.OP_STIFFNESS, tab, E=2e5, nu=0.3)
+    monkeypatch.setenv("LLFEM_PANEL_RUNTIME_STORES", "1")
+    loop, _ = dev.assemble(_lib.OP_STIFFNESS, tab, E=2e5, nu=0.3)
+    assert np.abs(fixed - loop).max() <= 1e-14 * np.abs(loop).max()
+
+
 def test_stiffness_matches_legacy_path(gpu_ctx):
     """GPU vs stiffnessMatrixSolid + sparse + dropzeros! (src/linear.jl:190-330):
     normwise 1e-12, as the reference's own DSL≡legacy test."""
