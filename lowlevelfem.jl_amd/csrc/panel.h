@@ -360,25 +360,16 @@ struct StepRec {
   int e, a, s;   // element, local index (-1: null step), panel row of node b
 };
 
+// Wide step records (rec_bytes per record: int element, int16 local index,
+// then npe map entries of MapT), decoded at load.
 template <int NPE, typename MapT>
 __device__ __forceinline__ StepRec load_rec(const void* recs, int rec_bytes, int64_t r, int b) {
   StepRec o;
-  if constexpr (NPE <= 11 && sizeof(MapT) == 1) {
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(recs) + r);
-    o.e = (int)v.x;
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-    const int byte = 5 + b;
-    const uint32_t a8 = v.y & 0xffu;
-    o.a = a8 == 0xffu ? -1 : (int)a8;
-    o.s = (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
-  } else {
-    const uint8_t* base = static_cast<const uint8_t*>(recs) + r * rec_bytes;
-    const int2 h = *reinterpret_cast<const int2*>(base);
-    o.e = h.x;
-    o.a = (int)(int16_t)(h.y & 0xffff);
-    o.s = (int)reinterpret_cast<const MapT*>(base + 8)[b];
-  }
+  const uint8_t* base = static_cast<const uint8_t*>(recs) + r * rec_bytes;
+  const int2 h = *reinterpret_cast<const int2*>(base);
+  o.e = h.x;
+  o.a = (int)(int16_t)(h.y & 0xffff);
+  o.s = (int)reinterpret_cast<const MapT*>(base + 8)[b];
   return o;
 }
 
@@ -403,6 +394,28 @@ __device__ __forceinline__ int raw_slot(const RawRec& v, int b) {
   const int byte = 5 + b;
   return (int)((w[byte >> 2] >> ((byte & 3) * 8)) & 0xffu);
 }
+
+// Ring entry of the generic wave body: compact records stay raw until used,
+// others are decoded at load.
+template <int NPE, typename MapT, bool RAW = (NPE <= 11 && sizeof(MapT) == 1)>
+struct Ring {
+  using T = StepRec;
+  static __device__ __forceinline__ T load(const void* recs, int rec_bytes, int64_t r, int b) {
+    return load_rec<NPE, MapT>(recs, rec_bytes, r, b);
+  }
+  static __device__ __forceinline__ int e(const T& x) { return x.e; }
+  static __device__ __forceinline__ int a(const T& x) { return x.a; }
+  static __device__ __forceinline__ int s(const T& x, int) { return x.s; }
+};
+
+template <int NPE, typename MapT>
+struct Ring<NPE, MapT, true> {
+  using T = RawRec;
+  static __device__ __forceinline__ T load(const void* recs, int, int64_t r, int) { return load_raw(recs, r); }
+  static __device__ __forceinline__ int e(const T& x) { return raw_elem(x); }
+  static __device__ __forceinline__ int a(const T& x) { return raw_loc(x); }
+  static __device__ __forceinline__ int s(const T& x, int b) { return raw_slot(x, b); }
+};
 
 // Waves [w0, w0 + nw) of a panel launch run by one wavefront, LDS panel space
 // wl (P.lds_per_wave doubles).  Consecutive waves' records and node
@@ -452,16 +465,18 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
   int2 dcur = desc[lane_rec];
   int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
 
-  StepRec rec[2 * PF];
+  using RG = Ring<NPE, MapT>;
+  typename RG::T rec[2 * PF];
   Geo<Real> g[PF];
   TabBlk<Real> tb[PF];
 #pragma unroll
   for (int v = 0; v < 2 * PF; ++v)
-    rec[v] = load_rec<NPE, MapT>(recs, P.rec_bytes, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
+    rec[v] = RG::load(recs, P.rec_bytes, min(rbase + (int64_t)v * NPW + lane_rec, rlast), b);
 #pragma unroll
   for (int v = 0; v < PF; ++v) {
-    g[v] = load_geo<Real>(geo + (size_t)rec[v].e * kGeoStride);
-    tb[v] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, rec[v].a < 0 ? 0 : rec[v].a, b);
+    const int a = RG::a(rec[v]);
+    g[v] = load_geo<Real>(geo + (size_t)RG::e(rec[v]) * kGeoStride);
+    tb[v] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, a < 0 ? 0 : a, b);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -472,11 +487,11 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
     for (int u = 0; u < 2 * PF; ++u) {
       const int64_t s = s0 + u;
       if (s < nsteps) {
-        const StepRec& cr = rec[u];
-        const bool live = grp && cr.a >= 0 && dcur.y > 0;
+        const int ca = RG::a(rec[u]);
+        const bool live = grp && ca >= 0 && dcur.y > 0;
         contribute<NPE, MapT, OP, AFFINE, ISO, Real>(P, geo, tab, tb[u % PF], live ? pan : dummy,
-                                                     0, live ? dcur.y : 0, b, cr.e,
-                                                     cr.a < 0 ? 0 : cr.a, live ? cr.s : 0,
+                                                     0, live ? dcur.y : 0, b, RG::e(rec[u]),
+                                                     ca < 0 ? 0 : ca, live ? RG::s(rec[u], b) : 0,
                                                      g[u % PF], live ? mofs : 0);
         if (++t == cnt) {   // round complete: write the NPW panels, clear them
           t = 0;
@@ -557,10 +572,12 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
         }
       }
       // ring refill: geometry/table for step s+PF, record for step s+2PF
-      const StepRec& nx = rec[(u + PF) % (2 * PF)];
-      g[u % PF] = load_geo<Real>(geo + (size_t)nx.e * kGeoStride);
-      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, nx.a < 0 ? 0 : nx.a, b);
-      rec[u] = load_rec<NPE, MapT>(recs, P.rec_bytes, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
+      const typename RG::T& nx = rec[(u + PF) % (2 * PF)];
+      const int na = RG::a(nx);
+      g[u % PF] = load_geo<Real>(geo + (size_t)RG::e(nx) * kGeoStride);
+      tb[u % PF] = load_tab<NPE, OP, AFFINE, Real>(tab, tabf, tab2, na < 0 ? 0 : na, b);
+      rec[u] = RG::load(recs, P.rec_bytes, min(rbase + (s + 2 * PF) * NPW + lane_rec, rlast), b);
+      __builtin_amdgcn_sched_barrier(0);   // keep the step's loads in the step
     }
   }
 }
