@@ -28,6 +28,7 @@
 // node-major adjacency).  The V-cycle kernels run table-driven over all
 // systems of a batch, one launch per kernel and level.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -443,6 +444,36 @@ __global__ void k_mg_strength(int64_t N, const int64_t* __restrict__ adj_ptr, co
 
 // ---- host hierarchy -------------------------------------------------------------
 
+}  // namespace
+
+// Host allocator for the large arrays of the hierarchy build: resize() leaves
+// the elements uninitialised, so the first touch of a large array happens
+// where it is filled -- in parallel threads -- rather than in a serial zero
+// fill.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  using value_type = T;
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using HVec = std::vector<T, NoInitAlloc<T>>;
+
+namespace {
+
 struct HostLevel {
   int64_t N = 0;                   // pseudo-nodes
   std::vector<int64_t> adj_ptr;    // level >= 1 only (level 0: the mesh's)
@@ -451,11 +482,11 @@ struct HostLevel {
   // transfer (all but the coarsest)
   int64_t nagg = 0;
   std::vector<int32_t> agg;        // [N]
-  std::vector<float> Q;            // [N][18]
+  HVec<float> Q;                   // [N][18]
   std::vector<int32_t> mem_ptr, mem;
   std::vector<int32_t> pair_o;     // [npairs][2]
   std::vector<int64_t> gptr;
-  std::vector<int32_t> gblk, gcol;
+  HVec<int32_t> gblk, gcol;
 };
 
 // A graph over real nodes: the neighbours of i are idx[e], e in
@@ -603,8 +634,9 @@ struct AmgHost {
 struct AmgInput {
   int64_t N = 0;
   std::vector<int64_t> adj_ptr;
-  std::vector<int32_t> adj, dslot;
-  std::vector<uint8_t> strong;     // per block
+  HVec<int32_t> adj;
+  std::vector<int32_t> dslot;
+  HVec<uint8_t> strong;            // per block
   std::vector<double> xyz;         // SoA
   std::vector<uint8_t> mask;       // [3N]
 };
@@ -663,7 +695,8 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
       if (L.agg[j] >= 0) L.mem[fill[L.agg[j]]++] = (int32_t)j;
   }
   // tentative prolongator and coarse near-null space (aggregates in parallel)
-  L.Q.assign((size_t)L.N * 18, 0.f);
+  L.Q.resize((size_t)L.N * 18);
+  parallel_for(L.N, [&](int64_t lo, int64_t hi) { std::fill(L.Q.begin() + 18 * lo, L.Q.begin() + 18 * hi, 0.f); });
   Bc.assign((size_t)na * 36, 0.0);
   C.N = 2 * na;
   C.mask.assign((size_t)C.N * 3, 0);
@@ -688,61 +721,83 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
     }
   });
   // per aggregate: neighbour aggregates (every fine block couples), then the
-  // Galerkin contribution lists of its pairs, fine blocks in ascending order
+  // Galerkin contribution lists of its pairs, fine blocks in ascending order.
+  // Two passes over the aggregates in parallel (sizes, then contents at
+  // offsets from prefix sums): each aggregate's lists depend on it alone, so
+  // the result does not depend on the thread count.
   nbp.assign((size_t)na + 1, 0);
-  nbi.clear();
-  nbi.reserve((size_t)na * 32);
-  L.gptr.assign(1, 0);
-  L.gblk.clear();
-  L.gcol.clear();
-  L.gblk.reserve((size_t)adj_ptr[L.N]);
-  L.gcol.reserve((size_t)adj_ptr[L.N]);
-  std::vector<int32_t> stamp((size_t)na, -1), pos((size_t)na, 0);
-  std::vector<int64_t> cur;
-  for (int64_t a = 0; a < na; ++a) {
-    const size_t b0 = nbi.size();
+  std::vector<int64_t> nblk((size_t)na + 1, 0);
+  // neighbour aggregates of a, ascending, into nb (stamp: per-thread scratch)
+  auto neighbours = [&](int64_t a, std::vector<int32_t>& stamp, std::vector<int32_t>& nb) {
+    nb.clear();
     stamp[a] = (int32_t)a;
-    nbi.push_back((int32_t)a);
+    nb.push_back((int32_t)a);
     for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
       const int64_t j = L.mem[m];
       for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
         const int32_t B = L.agg[adj[g]];
         if (B >= 0 && stamp[B] != a) {
           stamp[B] = (int32_t)a;
-          nbi.push_back(B);
+          nb.push_back(B);
         }
       }
     }
-    std::sort(nbi.begin() + b0, nbi.end());
-    const size_t k1 = nbi.size() - b0;
-    nbp[a + 1] = (int64_t)nbi.size();
-    for (size_t k = 0; k < k1; ++k) pos[nbi[b0 + k]] = (int32_t)k;
-    cur.assign(k1 + 1, 0);
-    for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
-      const int64_t j = L.mem[m];
-      for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
-        const int32_t B = L.agg[adj[g]];
-        if (B >= 0) cur[pos[B] + 1]++;
+    std::sort(nb.begin(), nb.end());
+  };
+  parallel_for(na, [&](int64_t lo, int64_t hi) {
+    std::vector<int32_t> stamp((size_t)na, -1), nb;
+    for (int64_t a = lo; a < hi; ++a) {
+      neighbours(a, stamp, nb);
+      nbp[a + 1] = (int64_t)nb.size();
+      int64_t c = 0;
+      for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
+        const int64_t j = L.mem[m];
+        for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) c += L.agg[adj[g]] >= 0 ? 1 : 0;
       }
+      nblk[a + 1] = c;
     }
-    const int64_t base = L.gptr.back();
-    for (size_t k = 0; k < k1; ++k) {
-      cur[k + 1] += cur[k];
-      L.gptr.push_back(base + cur[k + 1]);
-    }
-    L.gblk.resize((size_t)(base + cur[k1]));
-    L.gcol.resize((size_t)(base + cur[k1]));
-    for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
-      const int64_t j = L.mem[m];
-      for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
-        const int32_t B = L.agg[adj[g]];
-        if (B < 0) continue;
-        const int64_t o = base + cur[pos[B]]++;
-        L.gblk[o] = (int32_t)g;
-        L.gcol[o] = (int32_t)j;
-      }
-    }
+  });
+  for (int64_t a = 0; a < na; ++a) {
+    nbp[a + 1] += nbp[a];
+    nblk[a + 1] += nblk[a];
   }
+  nbi.assign((size_t)nbp[na], 0);
+  L.gptr.assign((size_t)nbp[na] + 1, 0);
+  L.gblk.resize((size_t)nblk[na]);   // every element is written below
+  L.gcol.resize((size_t)nblk[na]);
+  parallel_for(na, [&](int64_t lo, int64_t hi) {
+    std::vector<int32_t> stamp((size_t)na, -1), pos((size_t)na, 0), nb;
+    std::vector<int64_t> cur;
+    for (int64_t a = lo; a < hi; ++a) {
+      neighbours(a, stamp, nb);
+      const size_t k1 = nb.size();
+      std::copy(nb.begin(), nb.end(), nbi.begin() + nbp[a]);
+      for (size_t k = 0; k < k1; ++k) pos[nb[k]] = (int32_t)k;
+      cur.assign(k1 + 1, 0);
+      for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
+        const int64_t j = L.mem[m];
+        for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
+          const int32_t B = L.agg[adj[g]];
+          if (B >= 0) cur[pos[B] + 1]++;
+        }
+      }
+      const int64_t base = nblk[a];
+      for (size_t k = 0; k < k1; ++k) {
+        cur[k + 1] += cur[k];
+        L.gptr[nbp[a] + k + 1] = base + cur[k + 1];
+      }
+      for (int64_t m = L.mem_ptr[a]; m < L.mem_ptr[a + 1]; ++m) {
+        const int64_t j = L.mem[m];
+        for (int64_t g = adj_ptr[j]; g < adj_ptr[j + 1]; ++g) {
+          const int32_t B = L.agg[adj[g]];
+          if (B < 0) continue;
+          const int64_t o = base + cur[pos[B]]++;
+          L.gblk[o] = (int32_t)g;
+          L.gcol[o] = (int32_t)j;
+        }
+      }
+    }
+  });
   // coarse pseudo-node pattern: 2A and 2A+1 both list [2B, 2B+1 for B in nb(A)]
   const int64_t npairs = nbp[na];
   C.adj_ptr.assign((size_t)C.N + 1, 0);
@@ -798,6 +853,20 @@ int amg_gather(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgInput*
 }
 
 // The aggregation hierarchy (CPU only, no device calls: may run on any thread).
+namespace {
+// every value of v in [lo, hi), checked across host threads
+template <class V>
+bool all_in(const V& v, int64_t lo, int64_t hi) {
+  std::atomic<bool> ok{true};
+  parallel_for((int64_t)v.size(), [&](int64_t b, int64_t e) {
+    bool o = true;
+    for (int64_t i = b; i < e; ++i) o &= v[i] >= lo && v[i] < hi;
+    if (!o) ok = false;
+  });
+  return ok;
+}
+}  // namespace
+
 int amg_build_host(const AmgInput& in, AmgHost* H) {
   auto t0 = std::chrono::steady_clock::now();
   const int64_t N = in.N;
@@ -863,12 +932,8 @@ int amg_build_host(const AmgInput& in, AmgHost* H) {
               (int64_t)L.gptr.size() == (int64_t)L.pair_o.size() / 2 + 1 &&
               L.gptr.back() == (int64_t)L.gblk.size() && L.gblk.size() == L.gcol.size() &&
               (int64_t)C.dslot.size() == C.N && (int64_t)C.mask.size() == 3 * C.N;
-    for (int32_t a : L.agg) ok = ok && a >= -1 && a < L.nagg;
-    for (int32_t j : L.mem) ok = ok && j >= 0 && j < L.N;
-    for (int32_t g : L.gblk) ok = ok && g >= 0 && g < nadj;
-    for (int32_t j : L.gcol) ok = ok && j >= 0 && j < L.N;
-    for (int32_t o : L.pair_o) ok = ok && o >= 0 && o + 1 < cnadj;
-    for (int32_t i : C.adj) ok = ok && i >= 0 && i < C.N;
+    ok = ok && all_in(L.agg, -1, L.nagg) && all_in(L.mem, 0, L.N) && all_in(L.gblk, 0, nadj) &&
+         all_in(L.gcol, 0, L.N) && all_in(L.pair_o, 0, cnadj - 1) && all_in(C.adj, 0, C.N);
     for (int64_t j = 0; j < C.N && ok; ++j)
       ok = C.adj_ptr[j] <= C.adj_ptr[j + 1] && C.dslot[j] >= 0 && C.dslot[j] < C.adj_ptr[j + 1] - C.adj_ptr[j] &&
            C.adj[C.adj_ptr[j] + C.dslot[j]] == j;

@@ -46,9 +46,15 @@ int main(int argc, char** argv) {
   const std::string d = argc > 1 ? argv[1] : ".";
   llfem::AmgInput in;
   in.adj_ptr = rd<int64_t>(d + "/adj_ptr.bin");
-  in.adj = rd<int32_t>(d + "/adj.bin");
+  {
+    const std::vector<int32_t> a = rd<int32_t>(d + "/adj.bin");
+    in.adj.assign(a.begin(), a.end());
+  }
   in.dslot = rd<int32_t>(d + "/dslot.bin");
-  in.strong = rd<uint8_t>(d + "/strong.bin");
+  {
+    const std::vector<uint8_t> a = rd<uint8_t>(d + "/strong.bin");
+    in.strong.assign(a.begin(), a.end());
+  }
   in.xyz = rd<double>(d + "/xyz.bin");
   in.mask = rd<uint8_t>(d + "/mask.bin");
   in.N = (int64_t)in.adj_ptr.size() - 1;
@@ -61,7 +67,7 @@ int main(int argc, char** argv) {
     const llfem::HostLevel& F = H.lev[l];
     const llfem::HostLevel& C = H.lev[l + 1];
     const std::vector<int64_t>& ap = l == 0 ? in.adj_ptr : F.adj_ptr;
-    const std::vector<int32_t>& aj = l == 0 ? in.adj : F.adj;
+    const int32_t* aj = l == 0 ? in.adj.data() : F.adj.data();
     // membership
     std::vector<int> seen(F.N, 0);
     for (int64_t a = 0; a < F.nagg; ++a)
