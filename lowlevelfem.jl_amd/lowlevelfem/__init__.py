@@ -45,7 +45,7 @@ __all__ = [
     "structured_box_mesh", "stiffnessMatrix", "massMatrix", "loadVector", "solveDisplacement",
     "displacementConstraint", "load", "constrainedDoFs", "freeDoFs", "allDoFs",
     "nodesOnPhysicalGroup", "build_csc_pattern", "SymGrad", "Id", "integral", "dropzeros",
-    "elastic_constants", "Mesh",
+    "elastic_constants", "Mesh", "read_msh",
 ]
 
 _current_mesh: Mesh | None = None
@@ -56,6 +56,17 @@ def structured_box_mesh(**kw) -> Mesh:
     reference's gmsh.model.add("structured_box"), src/general.jl:8910)."""
     global _current_mesh
     _current_mesh = _sbm(**kw)
+    return _current_mesh
+
+
+def read_msh(path) -> Mesh:
+    """Read a gmsh MSH 4.1 ASCII file and make it the current model (the
+    reference's gmsh.open(path); node tags 1..N are the DoF base,
+    src/multifield.jl:1602-1611)."""
+    from .msh import read_msh as _read
+
+    global _current_mesh
+    _current_mesh = _read(path)
     return _current_mesh
 
 

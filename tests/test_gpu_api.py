@@ -102,3 +102,32 @@ def test_mirror_errors():
         lf.loadVector(P, lf.load("right", fy=1.0))
     with pytest.raises(NotImplementedError):
         lf.integral(lf.Id(P), np.eye(3), lf.Id(P))
+
+
+def test_reference_benchmark_mesh_file():
+    """The reference's own benchmark (lowlevelfem_elasticity_benchmark.jl:26-48)
+    on the mesh file gmsh wrote for it, through the mirror: K on gmsh's
+    numbering and its recorded checksum sum(abs, K) = 4.0073076923076982e9,
+    the pattern bit-exact against build_csc_pattern of the oracle, and the
+    solve with the left face clamped and traction (1, 0, 0) on the right,
+    whose recorded sum(u) is 0.07057241535937334 (results.txt:7)."""
+    import os
+
+    from oracle import oracle as O
+
+    say("[api] reference benchmark mesh file")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "elasticity_benchmark_n30.msh.gz")
+    m = lf.read_msh(path)
+    mat = lf.Material("body")
+    P = lf.Problem([mat])
+    K = lf.integral(lf.SymGrad(P), _D(mat), lf.SymGrad(P), csc_matrix=lf.build_csc_pattern(P))
+    cp, rv = O.csc_pattern(m)
+    assert np.array_equal(K.A.indptr, cp) and np.array_equal(K.A.indices, rv)
+    say(f"  sum(abs, K) {np.abs(K.A.data).sum()!r}")
+    assert abs(np.abs(K.A.data).sum() / 4.0073076923076982e9 - 1) < 1e-12
+    f = lf.loadVector(P, [lf.load("right", fx=1.0)])
+    support = [lf.displacementConstraint("left", ux=0.0, uy=0.0, uz=0.0)]
+    u = lf.solveDisplacement(K, f, support=support)
+    say(f"  sum(u) {u.a[:, 0].sum()!r}")
+    assert abs(u.a[:, 0].sum() / 0.07057241535937334 - 1) < 1e-10
+    assert np.all(u.a[lf.constrainedDoFs(P, support), 0] == 0.0)

@@ -1,0 +1,71 @@
+"""The MSH 4.1 reader on the reference's own benchmark mesh
+(examples/dev/compare_LLFEM_Ferrite_dealII_Gridap/elasticity_benchmark.msh,
+written by gmsh for structured_box_mesh(n = 30), gzip-compressed under
+tests/golden/).  Pins: the counts gmsh reported reading it (results.txt:10-12:
+29,791 nodes, 32,400 elements = 27,000 hex8 + 6 x 900 quads); the node set
+equals the 31^3 lattice of the unit cube; every hex has a positive Jacobian
+with the refelem hex8 order; the structural nnz is 9 (3n+1)^3 for n = 30;
+and the oracle's stiffness on gmsh's own numbering reproduces the
+reference's recorded sum(abs, K) = 4.0073076923076982e9 (results.txt:7)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import hooke_symgrad_D
+from lowlevelfem import mesh as M
+from lowlevelfem import refelem as R
+from lowlevelfem.msh import read_msh
+from oracle import oracle as O
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "elasticity_benchmark_n30.msh.gz")
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    return read_msh(FIXTURE)
+
+
+def test_counts_and_groups(mesh):
+    assert (mesh.N, mesh.E, mesh.etype, mesh.npe) == (29791, 27000, 5, 8)
+    assert sorted(mesh.faces) == ["bottom", "front", "left", "rear", "right", "top"]
+    for name, fs in mesh.faces.items():
+        assert fs.etype == 3 and fs.conn.shape == (900, 4), name
+    assert mesh.nodes_on("left").size == 31 * 31
+    assert np.allclose(mesh.coords[mesh.nodes_on("left"), 0], 0.0)
+    assert np.allclose(mesh.coords[mesh.nodes_on("right"), 0], 1.0)
+
+
+def test_node_set_is_the_lattice(mesh):
+    ref = M.structured_box_mesh(n=30).coords
+    a = np.round(mesh.coords * 30).astype(np.int64)
+    assert np.allclose(mesh.coords * 30, a, atol=1e-9)
+    b = np.round(ref * 30).astype(np.int64)
+    key = lambda p: np.sort(p[:, 0] * 10000 + p[:, 1] * 100 + p[:, 2])
+    assert np.array_equal(key(a), key(b))
+
+
+def test_hex_orientation(mesh):
+    dN = R.rule_tables(5, 1).dN[0]                       # (8, 3) at the cell centre
+    X = mesh.coords[mesh.conn]                           # (E, 8, 3)
+    J = np.einsum("eai,ak->eik", X, dN)
+    assert np.all(np.linalg.det(J) > 0)
+
+
+def test_stiffness_checksum_on_gmsh_numbering(mesh):
+    cp, rv = O.csc_pattern(mesh)
+    assert cp[-1] == 9 * 91 ** 3
+    K = O.stiffness_csc(mesh, R.rule_tables(5, 3), hooke_symgrad_D(), cp, rv, nthreads=4)
+    assert abs(np.abs(K).sum() / 4.0073076923076982e9 - 1) < 1e-12
+
+
+def test_reader_errors(tmp_path):
+    bad = tmp_path / "old.msh"
+    bad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
+    with pytest.raises(NotImplementedError):
+        read_msh(bad)
+    gap = tmp_path / "gap.msh"
+    gap.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n1 2 1 3\n3 1 0 2\n1\n3\n0 0 0\n1 0 0\n"
+                   "$EndNodes\n$Elements\n0 0 0 0\n$EndElements\n")
+    with pytest.raises(ValueError, match="node tags must be 1..N"):
+        read_msh(gap)
