@@ -156,6 +156,24 @@ LLFEM_API int llfem_load_traction(llfem_mesh* mesh, int32_t face_type, int32_t n
                         const double* N_ref, const double* dN_ref, const double traction[3],
                         double* f_out);
 
+/* Physical coordinates of the face Gauss points, x_q = sum_a N_a(xi_q) x_a
+ * (the x, y, z the reference evaluates a function-valued load at,
+ * src/linear.jl:3278-3280): xyz_out is [F][q][3].  With
+ * llfem_load_traction_gp this replaces loadVector with fx/fy/fz given as
+ * functions of (x, y, z) (_loadvec_helper, src/linear.jl:3288-3299): the
+ * caller evaluates its functions at these points. */
+LLFEM_API int llfem_face_points(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
+                                int64_t F, int32_t one_based, int32_t q, const double* N_ref,
+                                double* xyz_out);
+
+/* Traction given per face Gauss point, t_gp [F][q][3]:
+ * f_a += sum_q N_a t_q |t1 x t2| w_q (src/linear.jl:3378-3413), summed per
+ * node in ascending face order; f_out is [pdim*N], zero-initialised here. */
+LLFEM_API int llfem_load_traction_gp(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
+                                     int64_t F, int32_t one_based, int32_t q, const double* weights,
+                                     const double* N_ref, const double* dN_ref, const double* t_gp,
+                                     double* f_out);
+
 /* ---- solve -------------------------------------------------------------
  * Replaces solveDisplacement(K, f; support, iterative=true, reltol, maxiter,
  * preconditioner) (src/linear.jl:3711-3743) on the stiffness most recently

@@ -382,9 +382,25 @@ def massMatrix(problem: Problem, lumped=True, gauss=None) -> SystemMatrix | None
     return SystemMatrix(dropzeros(_csc(n, colptr, rowval, val)), problem, st.as_dict())
 
 
+def _eval_at(fn, P):
+    """A load component at the Gauss points P (F, q, 3): a number, or a
+    function of (x, y, z) as _loadvec_helper calls it (src/linear.jl:3288-3299);
+    called once on whole arrays when it accepts them, else point by point."""
+    shape = P.shape[:2]
+    if not callable(fn):
+        return np.full(shape, float(fn))
+    try:
+        v = np.asarray(fn(P[..., 0], P[..., 1], P[..., 2]), dtype=np.float64)
+        return np.broadcast_to(v, shape).copy()
+    except Exception:
+        flat = P.reshape(-1, 3)
+        return np.array([float(fn(*x)) for x in flat]).reshape(shape)
+
+
 def loadVector(problem: Problem, loads) -> VectorField | None:
-    """Traction loads load(name; fx, fy, fz) with constant components
-    (src/linear.jl:3088-3434)."""
+    """Traction loads load(name; fx, fy, fz) (src/linear.jl:3088-3434); each
+    component a number or a function of (x, y, z), evaluated at the face
+    Gauss points the library returns."""
     if problem.type == "dummy":
         return None
     if not isinstance(loads, (list, tuple)):
@@ -394,19 +410,22 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
     for ld in loads:
         if ld.problem is not None and ld.problem is not problem:
             continue
-        t = np.zeros(3)
+        comps = [0.0, 0.0, 0.0]
         for k, v in ld.values.items():
             if k not in ("fx", "fy", "fz"):
                 raise NotImplementedError(f"loadVector: load key {k} is not on the GPU path")
-            if callable(v):
-                raise NotImplementedError("loadVector: function-valued loads are not on the GPU path")
-            t["xyz".index(k[1])] = float(v)
+            comps["xyz".index(k[1])] = v
         if ld.phName not in problem.mesh.faces:
             raise KeyError(f"Physical group '{ld.phName}' not found.")
         fs = problem.mesh.faces[ld.phName]
         name, dim, order, npe, family = refelem.element_properties(fs.etype)
         tab = refelem.rule_tables(fs.etype, 2 * order + 1)
-        f += dev.traction(fs.etype, fs.conn, tab, t)
+        if any(callable(c) for c in comps):
+            P = dev.face_points(fs.etype, fs.conn, tab)
+            T = np.stack([_eval_at(c, P) for c in comps], axis=-1)
+            f += dev.traction_gp(fs.etype, fs.conn, tab, T)
+        else:
+            f += dev.traction(fs.etype, fs.conn, tab, np.array([float(c) for c in comps]))
     return VectorField(f.reshape(-1, 1), problem)
 
 

@@ -21,7 +21,8 @@ PCG_JACOBI_FP64, PCG_MIXED_REFINE, PCG_MIXED_RELIABLE, PCG_F32OP_RELIABLE, PCG_A
 EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
     "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
-    "llfem_assemble", "llfem_load_traction", "llfem_solve", "llfem_solve_csc",
+    "llfem_assemble", "llfem_load_traction", "llfem_face_points", "llfem_load_traction_gp",
+    "llfem_solve", "llfem_solve_csc",
     "llfem_assemble_batch", "llfem_mesh_values", "llfem_autotune",
     "llfem_server_create", "llfem_server_submit", "llfem_server_wait", "llfem_server_destroy",
 ]
@@ -95,6 +96,8 @@ def load_library(path: str = LIB_PATH):
     lib.llfem_mesh_pattern.argtypes = [P, I32, I32, C.POINTER(I64), P, P]
     lib.llfem_assemble.argtypes = [P, C.POINTER(OpDesc), P, C.POINTER(Stats)]
     lib.llfem_load_traction.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, P]
+    lib.llfem_face_points.argtypes = [P, I32, I32, P, I64, I32, I32, P, P]
+    lib.llfem_load_traction_gp.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, P]
     lib.llfem_solve.argtypes = [P, P, P, I64, I32, P, I32, D, I64, P, C.POINTER(Stats)]
     lib.llfem_solve_csc.argtypes = [P, I64, P, P, P, I32, P, P, I64, P, I32, D, I64, P,
                                     C.POINTER(Stats)]
@@ -251,6 +254,29 @@ class DeviceMesh:
         check(self.ctx.lib.llfem_load_traction(self.h, int(face_type), fc.shape[1], ptr(fc),
                                                fc.shape[0], 0, tables.q, ptr(w), ptr(Nr), ptr(dN),
                                                ptr(tt), ptr(f)))
+        return f
+
+    def face_points(self, face_type, face_conn, tables):
+        """Physical coordinates of the face Gauss points, (F, q, 3)."""
+        fc = np.ascontiguousarray(face_conn, dtype=np.int64)
+        Nr = np.ascontiguousarray(tables.N)
+        out = np.empty((fc.shape[0], tables.q, 3), dtype=np.float64)
+        check(self.ctx.lib.llfem_face_points(self.h, int(face_type), fc.shape[1], ptr(fc), fc.shape[0], 0,
+                                             tables.q, ptr(Nr), ptr(out)))
+        return out
+
+    def traction_gp(self, face_type, face_conn, tables, t_gp):
+        """Load vector of a traction given per face Gauss point, t_gp (F, q, 3)."""
+        fc = np.ascontiguousarray(face_conn, dtype=np.int64)
+        tg = np.ascontiguousarray(t_gp, dtype=np.float64)
+        if tg.shape != (fc.shape[0], tables.q, 3):
+            raise ValueError(f"traction_gp: t_gp has shape {tg.shape}, expected {(fc.shape[0], tables.q, 3)}")
+        f = np.empty(3 * self.N, dtype=np.float64)
+        w = np.ascontiguousarray(tables.weights)
+        Nr = np.ascontiguousarray(tables.N)
+        dN = np.ascontiguousarray(tables.dN)
+        check(self.ctx.lib.llfem_load_traction_gp(self.h, int(face_type), fc.shape[1], ptr(fc), fc.shape[0], 0,
+                                                  tables.q, ptr(w), ptr(Nr), ptr(dN), ptr(tg), ptr(f)))
         return f
 
     def solve(self, f, fixed, ufix=None, method=PCG_JACOBI_FP64, reltol=1e-10, maxiter=100000):

@@ -154,6 +154,43 @@ def traction(mesh, faceset, tab, t):
     return f
 
 
+def face_points(mesh, faceset, tab):
+    """Physical Gauss-point coordinates of every face element, (F, q, 3):
+    x = h[:, j]' * ncoord (src/linear.jl:3278-3280)."""
+    X = mesh.coords[faceset.conn]                       # (F, nf, 3)
+    return np.einsum("qa,fai->fqi", tab.N, X)
+
+
+def traction_fn(mesh, faceset, tab, fns):
+    """loadVector with function-valued components (fx, fy, fz callables of
+    x, y, z, or numbers; None = 0), per face element and Gauss point as
+    src/linear.jl:3277-3408 does it: t from _loadvec_helper (:3288-3299),
+    Ja = |t1 x t2| (:3378-3384), f1 += H1' t Ja w, fp[nn2] += f1 in element
+    order.  Plain numpy loops over the face elements (small meshes)."""
+    P = face_points(mesh, faceset, tab)
+    F, q = P.shape[:2]
+    T = np.zeros((F, q, 3))
+    for c, fn in enumerate(fns):
+        if fn is None:
+            continue
+        for e in range(F):
+            for j in range(q):
+                T[e, j, c] = fn(*P[e, j]) if callable(fn) else float(fn)
+    f = np.zeros(3 * mesh.N)
+    for e in range(F):
+        nodes = faceset.conn[e]
+        X = mesh.coords[nodes]
+        f1 = np.zeros((len(nodes), 3))
+        for j in range(q):
+            t1 = tab.dN[j, :, 0] @ X
+            t2 = tab.dN[j, :, 1] @ X
+            Ja = np.linalg.norm(np.cross(t1, t2))
+            f1 += np.outer(tab.N[j], T[e, j]) * Ja * tab.weights[j]
+        for a, v in enumerate(nodes):
+            f[3 * v:3 * v + 3] += f1[a]
+    return f, T
+
+
 def pcg(K: sp.csc_matrix, f, fixed, ufix=None, reltol=1e-10, maxiter=100000):
     """solveDisplacement(K, f; support, iterative=true, Pl=Jacobi)
     (src/linear.jl:3711-3743) with IterativeSolvers.cg semantics."""

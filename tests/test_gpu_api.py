@@ -93,6 +93,26 @@ def test_mass_shapes(elem, order):
     assert np.allclose(Ml.A.diagonal(), np.asarray(M.A.sum(axis=1)).ravel(), rtol=1e-12, atol=0.0)
 
 
+def test_function_valued_load():
+    """load(name; fx = f(x, y, z)) through the mirror (src/linear.jl:3288-3299):
+    fx = y on the right face of the unit cube has resultant 1/2 and nodal
+    first moment 1/3; a function returning a constant equals the constant
+    load; scalar-only functions are evaluated point by point."""
+    say("[api] function-valued load")
+    m = lf.structured_box_mesh(n=3, order=2)
+    P = lf.Problem([lf.Material("body")])
+    f = lf.loadVector(P, [lf.load("right", fx=lambda x, y, z: y)]).a[:, 0]
+    assert abs(f[0::3].sum() - 0.5) < 1e-13 and abs(f[0::3] @ m.coords[:, 1] - 1 / 3) < 1e-13
+    assert np.all(f[1::3] == 0) and np.all(f[2::3] == 0)
+    import math
+    fs = lf.loadVector(P, [lf.load("top", fz=lambda x, y, z: math.cos(x))]).a[:, 0]
+    fv = lf.loadVector(P, [lf.load("top", fz=lambda x, y, z: np.cos(x))]).a[:, 0]
+    assert np.abs(fs - fv).max() <= 1e-15
+    c1 = lf.loadVector(P, [lf.load("left", fy=lambda x, y, z: -2.0)]).a[:, 0]
+    c2 = lf.loadVector(P, [lf.load("left", fy=-2.0)]).a[:, 0]
+    assert np.abs(c1 - c2).max() <= 1e-14
+
+
 def test_mirror_errors():
     lf.structured_box_mesh(n=2, order=1)
     P = lf.Problem([lf.Material("body")])

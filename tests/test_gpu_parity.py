@@ -153,6 +153,27 @@ def test_traction_parity(gpu_ctx, elem, order):
         assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("elem,order", [("hex", 1), ("tet", 2), ("hex", 2), ("prism", 1)])
+def test_traction_function_parity(gpu_ctx, elem, order):
+    """Function-valued traction (fx = f(x, y, z) etc.): the face Gauss-point
+    coordinates from llfem_face_points and the load from
+    llfem_load_traction_gp against the oracle's per-point restatement, on a
+    perturbed mesh so the face Jacobians vary."""
+    say(f"[traction fn] {elem}{order}")
+    m = M.structured_box_mesh(n=4, elem=elem, order=order, perturb=0.08, seed=11)
+    dev = _dev(gpu_ctx, m)
+    fns = [lambda x, y, z: np.sin(3 * y) + z, None, lambda x, y, z: x * y - 2 * z * z]
+    for name in ("right", "top", "front"):
+        fs = m.faces[name]
+        name_, dim, o, nf, fam = R.element_properties(fs.etype)
+        tab = R.rule_tables(fs.etype, 2 * o + 1)
+        P = dev.face_points(fs.etype, fs.conn, tab)
+        assert np.abs(P - O.face_points(m, fs, tab)).max() <= 1e-14
+        ref, T = O.traction_fn(m, fs, tab, fns)
+        got = dev.traction_gp(fs.etype, fs.conn, tab, T)
+        assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
 def test_solve_golden_sum_u_30(gpu_ctx):
     """sum(u) = 0.07057241535937334 (results.txt:7) on the GPU: assemble,
     traction, Dirichlet-eliminated PCG, both FP64 and mixed precision."""

@@ -176,6 +176,27 @@ def test_traction_resultant(elem, order):
     assert np.allclose(f.reshape(-1, 3).sum(0), 1.0, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("elem,order", [("hex", 1), ("hex", 2), ("tet", 2)])
+def test_traction_function_closed_forms(elem, order):
+    """Function-valued traction on the right face x = 1 of the unit cube:
+    fx = y gives the resultant int y dA = 1/2 and, because the face basis
+    reproduces linear functions (sum_a N_a y_a = y), the nodal first moment
+    sum_a f_x,a y_a = int y^2 dA = 1/3; fz = 2 + z gives 5/2 and
+    sum_a f_z,a z_a = int (2 + z) z dA = 4/3.  A constant function equals
+    the constant-traction restatement (oracle.c) to rounding."""
+    m = M.structured_box_mesh(n=3, elem=elem, order=order)
+    fs = m.faces["right"]
+    tab = R.rule_tables(fs.etype, 2 * order + 1)
+    f, T = O.traction_fn(m, fs, tab, [lambda x, y, z: y, None, lambda x, y, z: 2.0 + z])
+    fx, fz = f[0::3], f[2::3]
+    y, z = m.coords[:, 1], m.coords[:, 2]
+    assert abs(fx.sum() - 0.5) < 1e-13 and abs(fz.sum() - 2.5) < 1e-13
+    assert abs(fx @ y - 1 / 3) < 1e-13 and abs(fz @ z - 4 / 3) < 1e-13
+    assert np.allclose(O.face_points(m, fs, tab)[..., 0], 1.0)
+    fc, _ = O.traction_fn(m, fs, tab, [0.5, lambda x, y, z: -1.0, 2.0])
+    assert np.allclose(fc, O.traction(m, fs, tab, [0.5, -1.0, 2.0]), rtol=0, atol=1e-15)
+
+
 def test_traction_nodal_values_hex8():
     m = M.structured_box_mesh(n=4)
     fs = m.faces["right"]
