@@ -45,7 +45,7 @@ __all__ = [
     "structured_box_mesh", "stiffnessMatrix", "massMatrix", "loadVector", "solveDisplacement",
     "displacementConstraint", "load", "constrainedDoFs", "freeDoFs", "allDoFs",
     "nodesOnPhysicalGroup", "build_csc_pattern", "SymGrad", "Id", "integral", "dropzeros",
-    "elastic_constants", "Mesh", "read_msh",
+    "elastic_constants", "Mesh", "read_msh", "solveField", "Symmetric",
 ]
 
 _current_mesh: Mesh | None = None
@@ -481,3 +481,27 @@ def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=Fal
     out.a[:, 0] = u
     out.stats = st.as_dict()
     return out
+
+
+class SymmetricSystemMatrix:
+    """Symmetric(K): K flagged symmetric (the reference then solves with
+    CHOLMOD, src/multifield.jl:4166-4268); the stored K is symmetric here,
+    so the wrapper only carries it."""
+
+    def __init__(self, K: SystemMatrix, uplo="U"):
+        self.parent, self.uplo = K, uplo
+
+
+def Symmetric(K: SystemMatrix, uplo="U") -> SymmetricSystemMatrix:
+    return SymmetricSystemMatrix(K, uplo)
+
+
+def solveField(K, f: VectorField, support=(), iterative=False, reltol=math.sqrt(np.finfo(float).eps),
+               maxiter=None, preconditioner="jacobi", ordering=True) -> VectorField:
+    """solveField(K, f; support, ...) and solveField(Symmetric(K), f; ...)
+    (src/poisson.jl:1432-1546, src/multifield.jl:4166-4268): the same
+    Dirichlet elimination as solveDisplacement, on the same device solve."""
+    if isinstance(K, SymmetricSystemMatrix):
+        K = K.parent
+    return solveDisplacement(K, f, support=support, iterative=iterative, reltol=reltol, maxiter=maxiter,
+                             preconditioner=preconditioner, ordering=ordering)
