@@ -83,6 +83,14 @@ typedef struct {
   double E, nu, rho;
   int32_t lumped;          /* mass: row-sum diagonal (src/linear.jl:1999-2001) */
   int32_t precision;       /* LLFEM_FP64 | LLFEM_FP32_INT_FP64_SUM | LLFEM_BF16X3 */
+  const double* element_scale; /* [E] or NULL: element e's integrand is multiplied by
+                              element_scale[e] (finite, >= 0).  A problem with several
+                              material groups (src/linear.jl:190-330 loops over
+                              problem.material) is assembled as one call per group
+                              with 1 on the group's elements and 0 elsewhere, and
+                              the results added in group order -- the order in which
+                              the reference's sparse() combines the groups' triplets.
+                              llfem_assemble only (batch and server: NULL). */
 } llfem_op_desc;
 
 typedef struct {

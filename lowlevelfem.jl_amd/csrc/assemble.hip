@@ -241,7 +241,7 @@ int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
   return LLFEM_OK;
 }
 
-int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
+int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const double* d_scale) {
   llfem_ctx* ctx = m->ctx;
   const int npe = m->npe, q = d->q;
   const size_t geo_doubles = (size_t)kGeoStride * m->E * (affine ? 1 : q);
@@ -249,7 +249,7 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine) {
   const int64_t threads = affine ? m->E : m->E * q;
   k_geometry<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,
-      m->d_tab + (size_t)q * npe * 3, m->d_geo);
+      m->d_tab + (size_t)q * npe * 3, m->d_geo, d_scale);
   return LLFEM_OK;
 }
 
@@ -308,7 +308,21 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   LLFEM_HIP(hipMemcpyAsync(m->d_tab + off, tab.data(), sizeof(double) * tab.size(),
                            hipMemcpyHostToDevice, st));
   LLFEM_HIP(hipMemcpyAsync(d_tabf, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice, st));
-  LLFEM_TRY(run_geometry(m, d, affine));
+  // per-element integrand factors (material groups): uploaded for this call
+  double* d_scale = nullptr;
+  if (d->element_scale) {
+    for (int64_t e = 0; e < m->E; ++e)
+      if (!(d->element_scale[e] >= 0.0) || !std::isfinite(d->element_scale[e]))
+        return fail(LLFEM_EINVAL, "llfem_assemble: element_scale must be finite and >= 0");
+    LLFEM_TRY(dalloc(ctx, &d_scale, (size_t)m->E));
+    LLFEM_HIP(hipMemcpyAsync(d_scale, d->element_scale, sizeof(double) * m->E, hipMemcpyHostToDevice, st));
+  }
+  const int grc = run_geometry(m, d, affine, d_scale);
+  if (d_scale) {
+    LLFEM_HIP(hipStreamSynchronize(st));   // the geometry kernel has read it
+    drelease(ctx, d_scale, (size_t)m->E);
+  }
+  LLFEM_TRY(grc);
 
   double** outp = (op == LLFEM_OP_STIFFNESS) ? &m->d_valK : &m->d_valM;
   const int blk = (op == LLFEM_OP_STIFFNESS) ? 9 : 3;

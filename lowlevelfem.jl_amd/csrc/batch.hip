@@ -177,6 +177,8 @@ int check_desc(const llfem_op_desc* d) {
     return fail(LLFEM_EINVAL, "llfem_assemble_batch: bad quadrature tables");
   if (d->op == LLFEM_OP_MASS && !d->N_ref)
     return fail(LLFEM_EINVAL, "llfem_assemble_batch: mass needs N_ref");
+  if (d->element_scale)
+    return fail(LLFEM_EUNSUP, "llfem_assemble_batch: element_scale is for llfem_assemble only");
   return LLFEM_OK;
 }
 

@@ -102,18 +102,21 @@ __device__ __forceinline__ void geometry_store(double* g, const double* Mi, doub
   f4[2] = make_float4((float)Mi[8], (float)wd, 0.f, 0.f);
 }
 
+// scale (nullable): per-element factor of the integrand (material groups),
+// folded into wd.
 __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int q, int affine,
                                              const double* __restrict__ xyz,
                                              const int32_t* __restrict__ conn,
                                              const double* __restrict__ dN,
                                              const double* __restrict__ w, double* __restrict__ geo,
-                                             int64_t e) {
+                                             int64_t e, const double* __restrict__ scale = nullptr) {
   double J[9], Mi[9];
   const int np = affine ? 1 : q;
+  const double se = scale ? scale[e] : 1.0;
   for (int p = 0; p < np; ++p) {
     jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
     const double det = inv3(J, Mi);
-    const double wd = affine ? fabs(det) : fabs(det) * w[p];
+    const double wd = (affine ? fabs(det) : fabs(det) * w[p]) * se;
     geometry_store(geo + ((size_t)e * np + p) * kGeoStride, Mi, wd);
   }
 }
@@ -123,11 +126,11 @@ __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int 
 __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
                            const double* __restrict__ xyz, const int32_t* __restrict__ conn,
                            const double* __restrict__ dN, const double* __restrict__ w,
-                           double* __restrict__ geo) {
+                           double* __restrict__ geo, const double* __restrict__ scale) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (affine) {
     if (t >= E) return;
-    geometry_one(E, N, npe, q, 1, xyz, conn, dN, w, geo, t);
+    geometry_one(E, N, npe, q, 1, xyz, conn, dN, w, geo, t, scale);
     return;
   }
   if (t >= E * q) return;
@@ -136,7 +139,7 @@ __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
   double J[9], Mi[9];
   jacobian(E, N, npe, e, xyz, conn, dN + (size_t)p * npe * 3, J);
   const double det = inv3(J, Mi);
-  geometry_store(geo + (size_t)t * kGeoStride, Mi, fabs(det) * w[p]);
+  geometry_store(geo + (size_t)t * kGeoStride, Mi, fabs(det) * w[p] * (scale ? scale[e] : 1.0));
 }
 
 // ---------------------------------------------------------------------------

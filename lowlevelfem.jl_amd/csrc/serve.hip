@@ -556,6 +556,8 @@ int check_request(const llfem_request* r) {
   if (!(r->reltol > 0) || r->maxiter <= 0) return fail(LLFEM_EINVAL, "llfem_server_submit: bad tolerance");
   if (!r->stiffness.weights || !r->stiffness.dN_ref || r->stiffness.q <= 0)
     return fail(LLFEM_EINVAL, "llfem_server_submit: bad quadrature tables");
+  if (r->stiffness.element_scale)
+    return fail(LLFEM_EUNSUP, "llfem_server_submit: element_scale is for llfem_assemble only");
   return LLFEM_OK;
 }
 

@@ -265,24 +265,56 @@ def build_csc_pattern(Pu: Problem, Ps: Problem | None = None) -> sp.csc_matrix:
     return _csc(Pu.ndofs(), colptr, rowval, np.zeros(rowval.size))
 
 
+def _material_groups(problem: Problem):
+    """(material, element scale or None) per material group.  Each material
+    names a volume physical group (a generated box has the one group "body");
+    a single material whose group holds every element runs the plain
+    assembly, otherwise each group's elements get scale 1 and the others 0
+    (the reference loops over problem.material and their groups' elements,
+    src/linear.jl:190-330)."""
+    vols = problem.mesh.volumes
+    if len(problem.material) == 1:
+        name = problem.material[0].phName
+        if name in vols and len(np.unique(vols[name])) == problem.mesh.E:
+            return [(problem.material[0], None)]
+        if name not in vols:
+            if vols or name != "body":      # a generated box has the one volume group "body"
+                raise KeyError(f"Physical group '{name}' not found.")
+            return [(problem.material[0], None)]
+    out = []
+    for mat in problem.material:
+        if mat.phName not in vols:
+            raise KeyError(f"Physical group '{mat.phName}' not found.")
+        sc = np.zeros(problem.mesh.E)
+        sc[vols[mat.phName]] = 1.0
+        out.append((mat, sc))
+    return out
+
+
 def stiffnessMatrix(problem: Problem, elements=(), forceOneThread=True, gauss=None,
                     precision="fp64", dropzeros_=True) -> SystemMatrix | None:
-    """stiffnessMatrix(problem) (src/linear.jl:70-87): Hooke material of the
-    first material group, rule "Gauss"*(2*order+1), CSC with dropzeros!
-    applied (src/linear.jl:323-324)."""
+    """stiffnessMatrix(problem) (src/linear.jl:70-87): Hooke material of each
+    material group, rule "Gauss"*(2*order+1), CSC with dropzeros! applied
+    (src/linear.jl:323-324).  Several groups: one device assembly per group
+    (element_scale 1 on its elements, 0 elsewhere), summed in group order."""
     if problem.type == "dummy":
         return None
-    if len(problem.material) != 1:
-        raise NotImplementedError("one material group per problem on the GPU path")
-    m = problem.material[0]
     dev = problem.device()
     tab = _rule(problem, gauss)
-    val, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision])
+    groups = _material_groups(problem)
+    val = None
+    for m, sc in groups:
+        v, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision],
+                             element_scale=sc)
+        val = v if val is None else val + v
     colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
     A = _csc(problem.ndofs(), colptr, rowval, val)
     if dropzeros_:
         A = dropzeros(A)
-    return SystemMatrix(A, problem, _device_k(problem, st))
+    if len(groups) == 1:
+        return SystemMatrix(A, problem, _device_k(problem, st))
+    problem._k_version = getattr(problem, "_k_version", 0) + 1   # the device copy holds one group only
+    return SystemMatrix(A, problem, st.as_dict())
 
 
 def _device_k(problem: Problem, st) -> dict:
@@ -370,10 +402,12 @@ def massMatrix(problem: Problem, lumped=True, gauss=None) -> SystemMatrix | None
     """massMatrix(problem; lumped=true) (src/linear.jl:1843-2004)."""
     if problem.type == "dummy":
         return None
-    m = problem.material[0]
     dev = problem.device()
     tab = _rule(problem, gauss)
-    val, st = dev.assemble(_lib.OP_MASS, tab, rho=m.rho, lumped=lumped)
+    val = None
+    for m, sc in _material_groups(problem):
+        v, st = dev.assemble(_lib.OP_MASS, tab, rho=m.rho, lumped=lumped, element_scale=sc)
+        val = v if val is None else val + v
     n = problem.ndofs()
     if lumped:
         A = sp.diags(val, 0, shape=(n, n), format="csc")

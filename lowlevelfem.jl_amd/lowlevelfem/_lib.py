@@ -32,7 +32,7 @@ class OpDesc(C.Structure):
     _fields_ = [("op", C.c_int32), ("q", C.c_int32), ("weights", C.c_void_p),
                 ("N_ref", C.c_void_p), ("dN_ref", C.c_void_p), ("D", C.c_void_p),
                 ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
-                ("lumped", C.c_int32), ("precision", C.c_int32)]
+                ("lumped", C.c_int32), ("precision", C.c_int32), ("element_scale", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -164,15 +164,18 @@ def default_context() -> Context:
     return _default_ctx
 
 
-def make_desc(op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False, precision=FP64):
+def make_desc(op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False, precision=FP64,
+              element_scale=None):
     """OpDesc for llfem_assemble; returns (desc, arrays that must outlive it)."""
     w = np.ascontiguousarray(tables.weights, dtype=np.float64)
     Nr = np.ascontiguousarray(tables.N, dtype=np.float64)
     dN = np.ascontiguousarray(tables.dN, dtype=np.float64)
     Dm = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(6, 6)
+    sc = None if element_scale is None else np.ascontiguousarray(element_scale, dtype=np.float64)
     desc = OpDesc(op, tables.q, ptr(w).value, ptr(Nr).value, ptr(dN).value,
-                  None if Dm is None else ptr(Dm).value, E, nu, rho, int(lumped), precision)
-    return desc, (w, Nr, dN, Dm)
+                  None if Dm is None else ptr(Dm).value, E, nu, rho, int(lumped), precision,
+                  None if sc is None else ptr(sc).value)
+    return desc, (w, Nr, dN, Dm, sc)
 
 
 def assemble_batch(items):
@@ -207,6 +210,7 @@ class DeviceMesh:
                                     cn.shape[0], 3, 0, C.byref(h)))
         self.h = h
         self.N = xyz.shape[0]
+        self.E = cn.shape[0]
 
     def pattern(self, op=OP_STIFFNESS, with_arrays=True):
         lib = self.ctx.lib
@@ -220,8 +224,10 @@ class DeviceMesh:
         return colptr, rowval
 
     def assemble(self, op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False,
-                 precision=FP64, copy_out=True):
-        desc, _keep = make_desc(op, tables, E, nu, rho, D, lumped, precision)
+                 precision=FP64, copy_out=True, element_scale=None):
+        if element_scale is not None and len(element_scale) != self.E:
+            raise ValueError(f"element_scale has {len(element_scale)} entries, the mesh {self.E} elements")
+        desc, _keep = make_desc(op, tables, E, nu, rho, D, lumped, precision, element_scale)
         st = Stats()
         out = None
         if copy_out:

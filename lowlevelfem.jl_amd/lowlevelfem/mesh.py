@@ -42,6 +42,7 @@ class Mesh:
     conn: np.ndarray           # (E, npe) int64, 0-based node ids
     faces: dict = field(default_factory=dict)   # name -> FaceSet
     name: str = "structured_box"
+    volumes: dict = field(default_factory=dict)  # volume physical group -> element indices
 
     @property
     def N(self) -> int:
@@ -58,6 +59,8 @@ class Mesh:
     def nodes_on(self, group: str) -> np.ndarray:
         """Sorted unique node ids (0-based) of a physical group; the
         reference's nodesOnPhysicalGroup (src/general.jl:5249-5275)."""
+        if group in self.volumes:
+            return np.unique(self.conn[self.volumes[group]])
         if group == "body":
             return np.arange(self.N, dtype=np.int64)
         if group not in self.faces:

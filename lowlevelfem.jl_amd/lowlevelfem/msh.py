@@ -10,7 +10,8 @@ This module reads the published MSH 4.1 ASCII layout directly -- sections
 same ``Mesh`` the structured generators build: node ``tag - 1`` is row
 ``tag - 1`` of the coordinates (gmsh's own numbering, so the CSC pattern is
 the one the reference builds on this file), the volume elements in file order
-with gmsh's node order, and the elements of every 2D physical group as a
+with gmsh's node order, the element indices of every 3D physical group (the
+material groups), and the elements of every 2D physical group as a
 ``FaceSet`` under the group's name.  Files ending in ``.gz`` are read through
 gzip.
 
@@ -138,6 +139,15 @@ def read_msh(path) -> Mesh:
     if vtype not in GMSH_ORDERED or re.element_properties(vtype)[1] != 3:
         raise NotImplementedError(f"read_msh: volume element type {vtype} is not supported")
     conn = np.ascontiguousarray(np.concatenate([b[3] for b in vol]))
+    # volume physical groups: element indices (file order) of their entities
+    first = np.cumsum([0] + [b[3].shape[0] for b in vol])
+    volumes = {}
+    for (dim, ptag), name in names.items():
+        if dim != 3:
+            continue
+        idx = [np.arange(first[i], first[i + 1]) for i, b in enumerate(vol) if ptag in ent_phys.get((3, b[1]), [])]
+        if idx:
+            volumes[name] = np.concatenate(idx)
 
     faces = {}
     for (dim, ptag), name in names.items():
@@ -153,4 +163,4 @@ def read_msh(path) -> Mesh:
     if conn.min() < 0 or conn.max() >= nnodes:
         raise ValueError("read_msh: element node tag out of range")
     return Mesh(coords=coords, etype=vtype, conn=conn, faces=faces,
-                name=os.path.basename(str(path)).split(".")[0])
+                name=os.path.basename(str(path)).split(".")[0], volumes=volumes)

@@ -14,6 +14,8 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import lowlevelfem as lf
+from lowlevelfem import _lib
+from lowlevelfem import refelem as R
 
 pytestmark = pytest.mark.gpu
 
@@ -153,3 +155,50 @@ def test_reference_benchmark_mesh_file():
     assert np.all(u.a[lf.constrainedDoFs(P, support), 0] == 0.0)
     u2 = lf.solveDisplacement(K, f, support=support)
     assert np.array_equal(u2.a, u.a)
+
+
+def test_several_material_groups():
+    """Problem([Material("steel"; ...), Material("alu"; ...)]) on two volume
+    groups (src/linear.jl:190-330 loops over problem.material): stiffness and
+    consistent mass against the oracle assembled group by group and summed in
+    group order; one material on every element equals the plain assembly;
+    an unknown group name fails as the reference does."""
+    import copy
+
+    from oracle import oracle as O
+
+    say("[api] several material groups")
+    m = lf.structured_box_mesh(n=4, elem="tet", order=2)
+    xc = m.coords[m.conn].mean(axis=1)[:, 0]
+    m.volumes = {"steel": np.nonzero(xc < 0.5)[0], "alu": np.nonzero(xc >= 0.5)[0]}
+    steel = lf.Material("steel", E=2.1e5, nu=0.3, rho=7.85e-9)
+    alu = lf.Material("alu", E=7.0e4, nu=0.34, rho=2.7e-9)
+    P = lf.Problem([steel, alu])
+    K = lf.stiffnessMatrix(P, dropzeros_=False)
+    M = lf.massMatrix(P, lumped=False)
+    cp, rv = O.csc_pattern(m)
+    tab = R.rule_tables(m.etype, 5)
+    Kref = np.zeros(cp[-1])
+    Mref = None
+    for mat in (steel, alu):
+        sub = copy.copy(m)
+        sub.conn = m.conn[m.volumes[mat.phName]]
+        Kref = Kref + O.stiffness_csc(sub, tab, _D(mat), cp, rv)
+        Mg = O.mass_legacy(sub, tab, rho=mat.rho)
+        Mref = Mg if Mref is None else Mref + Mg
+    assert np.array_equal(K.A.indptr, cp) and np.array_equal(K.A.indices, rv)
+    assert np.abs(K.A.data - Kref).max() <= 1e-12 * np.abs(Kref).max()
+    assert spla.norm(M.A - Mref) / spla.norm(Mref) < 1e-12
+    # one material named for a group that holds every element: the plain path
+    m.volumes["all"] = np.arange(m.E)
+    P1 = lf.Problem([lf.Material("all", E=2.1e5, nu=0.3)])
+    K1 = lf.stiffnessMatrix(P1, dropzeros_=False)
+    P2 = lf.Problem([lf.Material("body", E=2.1e5, nu=0.3)])
+    del m.volumes["all"]
+    with pytest.raises(KeyError, match="Physical group 'body' not found"):
+        lf.stiffnessMatrix(P2)
+    m.volumes = {}
+    K2 = lf.stiffnessMatrix(lf.Problem([lf.Material("body", E=2.1e5, nu=0.3)]), dropzeros_=False)
+    assert np.array_equal(K1.A.data, K2.A.data)
+    with pytest.raises(_lib.LLFEMError, match="element_scale must be finite and >= 0"):
+        K1.model.device().assemble(_lib.OP_STIFFNESS, tab, element_scale=-np.ones(m.E))
