@@ -123,9 +123,16 @@ class Problem:
     """A :Solid problem on a mesh (src/general.jl:478-632).  ``non`` is the
     node count; dim = pdim = 3."""
 
-    def __init__(self, materials, type="Solid", mesh: Mesh | None = None, context=None):
+    def __init__(self, materials, type="Solid", mesh: Mesh | None = None, context=None, dim=3,
+                 field="u", bandwidth=None):
+        # :VectorField with dim = 3 and field :u is the DSL's spelling of the
+        # same 3D vector problem (src/general.jl:478-632)
+        if type == "VectorField" and dim == 3 and field == "u":
+            type = "Solid"
         if type not in ("Solid", "dummy"):
             raise ValueError("only :Solid (3D linear elasticity) problems are on the GPU path")
+        if bandwidth not in (None, "none"):
+            raise NotImplementedError("Problem: node renumbering (bandwidth=...) is not on the GPU path")
         self.material = list(materials)
         self.type = type
         self.mesh = mesh if mesh is not None else _current_mesh
@@ -373,12 +380,13 @@ def _integral_id(a, c, b, gauss, csc_matrix):
     return SystemMatrix(K, P, st.as_dict())
 
 
-def integral(a, D, b, gauss=None, csc_matrix=None, precision="fp64"):
+def integral(a, D, b, gauss=None, csc_matrix=None, precision="fp64", threads=None):
     """∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); gauss, csc_matrix) (src/multifield.jl:5295-5339).
     D is the 6x6 coefficient in SymGrad row order [xx,yy,zz,xy,xz,yz].
     Without csc_matrix the result gets dropzeros! (:5336); with it the
     structural pattern is kept and the caller's matrix is filled in place.
-    integral(Id(Pu), c, Id(Pu), ...) is the mass-shaped form (scalar c)."""
+    integral(Id(Pu), c, Id(Pu), ...) is the mass-shaped form (scalar c).
+    ``threads`` (the reference's worker count) has no meaning on the GPU path."""
     if isinstance(a, Id) or isinstance(b, Id):
         return _integral_id(a, D, b, gauss, csc_matrix)
     if not isinstance(a, SymGrad) or not isinstance(b, SymGrad) or a.P is not b.P:

@@ -141,8 +141,8 @@ def test_reference_benchmark_mesh_file():
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "elasticity_benchmark_n30.msh.gz")
     m = lf.read_msh(path)
     mat = lf.Material("body")
-    P = lf.Problem([mat])
-    K = lf.integral(lf.SymGrad(P), _D(mat), lf.SymGrad(P), csc_matrix=lf.build_csc_pattern(P))
+    P = lf.Problem([mat], type="VectorField", dim=3, field="u")   # benchmark.jl:30, renumbering aside
+    K = lf.integral(lf.SymGrad(P), _D(mat), lf.SymGrad(P), csc_matrix=lf.build_csc_pattern(P), threads="auto")
     cp, rv = O.csc_pattern(m)
     assert np.array_equal(K.A.indptr, cp) and np.array_equal(K.A.indices, rv)
     say(f"  sum(abs, K) {np.abs(K.A.data).sum()!r}")
