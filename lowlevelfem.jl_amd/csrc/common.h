@@ -158,6 +158,8 @@ struct llfem_mesh {
   int64_t nmulti = 0;
   void* d_ho = nullptr;           // staged blocks + Z
   size_t ho_bytes = 0;
+  std::vector<int32_t> h_conn0;   // nodes of element 0 (0-based)
+  int32_t* d_ho_perm = nullptr;   // [npe] local node of each Gram row position (ascending node id)
   // AMG hierarchies kept per support (key: hash of the constrained-DoF list),
   // each with the device mask it was built for
   std::map<uint64_t, std::pair<llfem::AmgHier*, uint8_t*>> amg;

@@ -15,9 +15,11 @@
 // deterministic segmented gather (contributions in ascending element order).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 
 #include "common.h"
 
@@ -49,9 +51,13 @@ __device__ __forceinline__ float bf16_to_f(short h) {
 //   FP64  Z[e][p/4][row][p%4]           (k-blocks of 4: one MFMA fragment = 512 contiguous B)
 //   BF16  Zh/Zl[e][p/16][row][p%16]     (k-blocks of 16)
 // Padding rows (>= 3n) and points (>= q) are written as zero.
+// Row position b holds local node perm[b] (the element's nodes in ascending
+// global node id, so the Gram epilogue's neighbouring lanes write
+// neighbouring CSC slots).
 template <bool BF16>
 __global__ void k_hex_z(int64_t E, int npe, int q, int rows, int affine, const double* __restrict__ geo,
                         const double* __restrict__ dN, const double* __restrict__ w,
+                        const int32_t* __restrict__ perm,
                         double* __restrict__ Z, short* __restrict__ Zh, short* __restrict__ Zl) {
   const int64_t e = blockIdx.x;
   const int p = threadIdx.x;           // point (blockDim = qp)
@@ -67,7 +73,7 @@ __global__ void k_hex_z(int64_t E, int npe, int q, int rows, int affine, const d
   for (int b = 0; b < rows / 3; ++b) {
     double z[3] = {0.0, 0.0, 0.0};
     if (p < q && b < npe) {
-      const double* h = dN + ((size_t)p * npe + b) * 3;
+      const double* h = dN + ((size_t)p * npe + perm[b]) * 3;
 #pragma unroll
       for (int al = 0; al < 3; ++al) z[al] = sw * (Mi[al] * h[0] + Mi[3 + al] * h[1] + Mi[6 + al] * h[2]);
     }
@@ -104,6 +110,7 @@ struct GramParams {
   // a block that is the only contribution to its CSC block goes straight to
   // the CSC values (direct[staged index] = CSC block, -1: staged)
   const int32_t* direct;
+  const int32_t* perm;    // [npe] local node of each row position
   const int32_t* blk_node;
   const int64_t* adj_ptr;
   double* out;
@@ -172,8 +179,9 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, 
             K[d * 3 + c] = acc;
           }
       }
-      // block of column node cn, row node rn: [c][d] = K[d][c]
-      const int cn = pass == 0 ? a : b, rn = pass == 0 ? b : a;
+      // block of column node cn, row node rn (local nodes): [c][d] = K[d][c]
+      const int bl = P.perm[b], al_ = P.perm[a];
+      const int cn = pass == 0 ? al_ : bl, rn = pass == 0 ? bl : al_;
       const int32_t p = P.direct[((size_t)e * npe + cn) * npe + rn];
       if (p >= 0) {   // CSC panel of node j: value (c, slot s, d) at 9 a0 + c 3R + 3 s + d
         const int64_t j = P.blk_node[p];
@@ -503,12 +511,19 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   double* Z = reinterpret_cast<double*>(base + sd * sizeof(double));
   short* Zh = reinterpret_cast<short*>(Z);
   short* Zl = Zh + zd;
+  if (!m->d_ho_perm) {   // element 0's nodes in ascending global id
+    std::vector<int32_t> perm(npe);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return m->h_conn0[x] < m->h_conn0[y]; });
+    LLFEM_TRY(dalloc(ctx, &m->d_ho_perm, (size_t)npe));
+    LLFEM_HIP(hipMemcpyAsync(m->d_ho_perm, perm.data(), sizeof(int32_t) * npe, hipMemcpyHostToDevice, st));
+  }
   if (bf)
     k_hex_z<true><<<(unsigned)m->E, qp, 0, st>>>(m->E, npe, q, rows, affine ? 1 : 0, m->d_geo, d_dN, d_w,
-                                                 nullptr, Zh, Zl);
+                                                 m->d_ho_perm, nullptr, Zh, Zl);
   else
     k_hex_z<false><<<(unsigned)m->E, qp, 0, st>>>(m->E, npe, q, rows, affine ? 1 : 0, m->d_geo, d_dN, d_w,
-                                                  Z, nullptr, nullptr);
+                                                  m->d_ho_perm, Z, nullptr, nullptr);
   GramParams P{};
   P.E = m->E;
   P.npe = npe;
@@ -524,6 +539,7 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   if (C81) std::memcpy(P.C, C81, sizeof(P.C));
   P.S = S;
   P.direct = m->d_direct;
+  P.perm = m->d_ho_perm;
   P.blk_node = m->d_blk_node;
   P.adj_ptr = m->d_adj_ptr;
   P.out = out;

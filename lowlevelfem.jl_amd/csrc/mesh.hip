@@ -438,6 +438,7 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_direct, (size_t)m->E * m->npe * m->npe);
   drelease(c, m->d_multi, (size_t)std::max<int64_t>(m->nmulti, 1));
   if (m->d_ho) dfree(c, m->d_ho, m->ho_bytes);
+  drelease(c, m->d_ho_perm, (size_t)m->npe);
   for (auto& kv : m->amg) {
     amg_free(c, kv.second.first);
     dfree(c, kv.second.second, 3 * (size_t)m->N);
@@ -473,6 +474,8 @@ int llfem_mesh_create(llfem_ctx* ctx, const double* xyz, int64_t N, int32_t elem
   llfem_mesh* m = new llfem_mesh();
   m->ctx = ctx;
   m->N = N; m->E = E; m->npe = npe; m->etype = elem_type; m->pdim = pdim;
+  m->h_conn0.resize(npe);
+  for (int a = 0; a < npe; ++a) m->h_conn0[a] = h_conn[(size_t)a * E];
   int s = LLFEM_OK;
   double* d_aos = nullptr;
   do {
