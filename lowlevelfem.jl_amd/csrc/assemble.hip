@@ -419,13 +419,11 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   }
   if (nzval_out) {
     if (op == LLFEM_OP_MASS && d->lumped)
-      LLFEM_HIP(hipMemcpyAsync(nzval_out, m->d_lumped, sizeof(double) * 3 * m->N,
-                               hipMemcpyDeviceToHost, st));
+      LLFEM_TRY(copy_to_host(ctx, nzval_out, m->d_lumped, 3 * (size_t)m->N));
     else
-      LLFEM_HIP(hipMemcpyAsync(nzval_out, target, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
+      LLFEM_TRY(copy_to_host(ctx, nzval_out, target, (size_t)nnz));
   }
   LLFEM_HIP(hipEventSynchronize(ctx->ev[3]));
-  if (nzval_out) LLFEM_HIP(hipStreamSynchronize(st));
   if (stats) {
     float ms = 0.f, ms_main = 0.f;
     LLFEM_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));

@@ -489,9 +489,7 @@ extern "C" int llfem_mesh_values(llfem_mesh* m, int32_t op, int32_t lumped, doub
   } else {
     return fail(LLFEM_EINVAL, "llfem_mesh_values: unknown operator");
   }
-  LLFEM_HIP(hipMemcpyAsync(nzval_out, src, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
-  LLFEM_HIP(hipStreamSynchronize(ctx->stream));
-  return LLFEM_OK;
+  return copy_to_host(ctx, nzval_out, src, count);
 }
 
 extern "C" int llfem_autotune(llfem_mesh* m, const llfem_op_desc* d, llfem_tune_result* out) {

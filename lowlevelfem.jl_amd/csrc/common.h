@@ -98,7 +98,11 @@ struct llfem_ctx {
   std::map<uint64_t, llfem::DevTable> tables;      // by content hash
   void* d_scratch = nullptr;                       // batch descriptors + queue counter
   size_t scratch_bytes = 0;
-  double* d_sink = nullptr;                        // 64 doubles: target of padding-lane panel stores
+  double* d_sink = nullptr;                        // 128 doubles: target of padding-lane panel stores
+  // pinned staging for large device-to-host copies (copy_to_host)
+  void* h_stage[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct llfem_mesh {
@@ -168,6 +172,9 @@ struct llfem_mesh {
 namespace llfem {
 
 int dmalloc(llfem_ctx* ctx, void** p, size_t bytes);
+// n doubles from device memory to pageable host memory, after the work
+// already queued on the context's stream; returns when dst holds them.
+int copy_to_host(llfem_ctx* ctx, double* dst, const double* src, size_t n);
 void dfree(llfem_ctx* ctx, void* p, size_t bytes);
 
 template <typename T>

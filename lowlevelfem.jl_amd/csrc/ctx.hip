@@ -1,7 +1,10 @@
 // Context, error reporting and device-memory accounting.
 #include "common.h"
 
+#include <algorithm>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 namespace llfem {
 
@@ -31,6 +34,52 @@ void dfree(llfem_ctx* ctx, void* p, size_t bytes) {
   if (!p) return;
   (void)hipFree(p);
   ctx->in_use -= bytes;
+}
+
+// Large copies go through two pinned staging buffers: the DMA engine fills
+// one while host threads copy the other out to the caller's pageable memory
+// (HIP's own pageable path stages serially through one buffer).
+int copy_to_host(llfem_ctx* ctx, double* dst, const double* src, size_t n) {
+  constexpr size_t kStage = size_t(64) << 20;   // bytes per staging buffer
+  const size_t bytes = n * sizeof(double);
+  hipStream_t st = ctx->stream;
+  if (bytes <= 2 * kStage) {
+    LLFEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  }
+  if (!ctx->h_stage[0]) {
+    for (int b = 0; b < 2; ++b) {
+      LLFEM_HIP(hipHostMalloc(&ctx->h_stage[b], kStage, hipHostMallocDefault));
+      LLFEM_HIP(hipEventCreateWithFlags(&ctx->stage_ev[b], hipEventDisableTiming));
+    }
+    ctx->stage_bytes = kStage;
+  }
+  const size_t chunk = kStage / sizeof(double);
+  const size_t nchunks = (n + chunk - 1) / chunk;
+  auto issue = [&](size_t i) -> int {
+    const size_t off = i * chunk, len = std::min(chunk, n - off);
+    LLFEM_HIP(hipMemcpyAsync(ctx->h_stage[i & 1], src + off, len * sizeof(double), hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipEventRecord(ctx->stage_ev[i & 1], st));
+    return LLFEM_OK;
+  };
+  LLFEM_TRY(issue(0));
+  if (nchunks > 1) LLFEM_TRY(issue(1));
+  const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  for (size_t i = 0; i < nchunks; ++i) {
+    LLFEM_HIP(hipEventSynchronize(ctx->stage_ev[i & 1]));
+    const size_t off = i * chunk, len = std::min(chunk, n - off);
+    const double* from = static_cast<const double*>(ctx->h_stage[i & 1]);
+    std::vector<std::thread> th;
+    const size_t per = (len + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+      const size_t lo = t * per, hi = std::min(len, lo + per);
+      if (lo < hi) th.emplace_back([=] { std::memcpy(dst + off + lo, from + lo, (hi - lo) * sizeof(double)); });
+    }
+    for (auto& x : th) x.join();
+    if (i + 2 < nchunks) LLFEM_TRY(issue(i + 2));
+  }
+  return LLFEM_OK;
 }
 
 // Content hash for cache keys (not cryptographic): four independent 64-bit
@@ -103,7 +152,11 @@ int llfem_ctx_destroy(llfem_ctx* ctx) {
   for (auto& kv : ctx->tables) llfem::dfree(ctx, kv.second.ptr, kv.second.bytes);
   ctx->tables.clear();
   if (ctx->d_scratch) llfem::dfree(ctx, ctx->d_scratch, ctx->scratch_bytes);
-  if (ctx->d_sink) llfem::dfree(ctx, ctx->d_sink, 64 * sizeof(double));
+  if (ctx->d_sink) llfem::dfree(ctx, ctx->d_sink, 128 * sizeof(double));
+  for (int b = 0; b < 2; ++b) {
+    if (ctx->h_stage[b]) (void)hipHostFree(ctx->h_stage[b]);
+    if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LLFEM_OK;

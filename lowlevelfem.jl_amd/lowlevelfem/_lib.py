@@ -224,14 +224,18 @@ class DeviceMesh:
         return colptr, rowval
 
     def assemble(self, op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False,
-                 precision=FP64, copy_out=True, element_scale=None):
+                 precision=FP64, copy_out=True, element_scale=None, out=None):
+        """Values to a new host array (copy_out), to `out` (a float64 array of
+        the right size), or left on the device (copy_out=False)."""
         if element_scale is not None and len(element_scale) != self.E:
             raise ValueError(f"element_scale has {len(element_scale)} entries, the mesh {self.E} elements")
         desc, _keep = make_desc(op, tables, E, nu, rho, D, lumped, precision, element_scale)
         st = Stats()
-        out = None
-        if copy_out:
-            n = 3 * self.N if (op == OP_MASS and lumped) else self.pattern(op, with_arrays=False)
+        n = 3 * self.N if (op == OP_MASS and lumped) else self.pattern(op, with_arrays=False)
+        if out is not None:
+            if out.dtype != np.float64 or out.size != n or not out.flags.c_contiguous:
+                raise ValueError(f"assemble: out must be a contiguous float64 array of {n} values")
+        elif copy_out:
             out = np.empty(n, dtype=np.float64)
         check(self.ctx.lib.llfem_assemble(self.h, C.byref(desc), ptr(out), C.byref(st)))
         return out, st

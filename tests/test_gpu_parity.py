@@ -88,6 +88,21 @@ def test_fixed_store_count_path_is_bitwise_the_loop_path(gpu_ctx, monkeypatch, l
     assert np.abs(fixed - loop).max() <= 1e-14 * np.abs(loop).max()
 
 
+def test_large_values_copied_out_in_chunks(gpu_ctx):
+    """Values larger than two 64 MB pinned staging buffers reach the caller's
+    array chunk by chunk (4 chunks here); every entry against the oracle."""
+    say("[stiffness] chunked copy-out")
+    m = M.structured_box_mesh(lz=2, cells=(20, 20, 32), elem="tet", order=2)
+    tab = R.rule_tables(11, 2)
+    cp, rv = O.csc_pattern(m)
+    assert cp[-1] * 8 > 3 * (64 << 20)
+    ref = O.stiffness_csc(m, tab, hooke_symgrad_D(), cp, rv, nthreads=4)
+    dev = _dev(gpu_ctx, m)
+    val, _ = dev.assemble(_lib.OP_STIFFNESS, tab, E=2e5, nu=0.3)
+    _check_values(val, ref)
+    assert np.array_equal(dev.values(_lib.OP_STIFFNESS), val)
+
+
 def test_stiffness_matches_legacy_path(gpu_ctx):
     """GPU vs stiffnessMatrixSolid + sparse + dropzeros! (src/linear.jl:190-330):
     normwise 1e-12, as the reference's own DSL≡legacy test."""
