@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <numeric>
 
+#include <thread>
+
 #include "common.h"
 
 namespace llfem {
@@ -530,23 +532,34 @@ int llfem_mesh_pattern(llfem_mesh* m, int32_t op, int32_t one_based, int64_t* nn
     LLFEM_HIP(hipMemcpy(adj.data(), m->d_adj, sizeof(int32_t) * m->nadj, hipMemcpyDeviceToHost));
   }
   const auto& ap = m->h_adj_ptr;
-  for (int64_t j = 0; j < N; ++j) {
-    const int64_t R = ap[j + 1] - ap[j];
-    const int64_t p0 = ap[j] * blk;
-    for (int c = 0; c < pd; ++c) {
-      const int64_t pc = p0 + c * R * (blk / pd);
-      if (colptr) colptr[j * pd + c] = pc + base;
-      if (rowval) {
-        if (op == LLFEM_OP_MASS) {
-          for (int64_t s = 0; s < R; ++s) rowval[pc + s] = (int64_t)adj[ap[j] + s] * pd + c + base;
-        } else {
-          for (int64_t s = 0; s < R; ++s)
-            for (int d = 0; d < pd; ++d)
-              rowval[pc + s * pd + d] = (int64_t)adj[ap[j] + s] * pd + d + base;
+  // columns of disjoint node ranges fill disjoint parts of colptr / rowval:
+  // host threads take contiguous node ranges
+  auto fill = [&](int64_t j0, int64_t j1) {
+    for (int64_t j = j0; j < j1; ++j) {
+      const int64_t R = ap[j + 1] - ap[j];
+      const int64_t p0 = ap[j] * blk;
+      for (int c = 0; c < pd; ++c) {
+        const int64_t pc = p0 + c * R * (blk / pd);
+        if (colptr) colptr[j * pd + c] = pc + base;
+        if (rowval) {
+          if (op == LLFEM_OP_MASS) {
+            for (int64_t s = 0; s < R; ++s) rowval[pc + s] = (int64_t)adj[ap[j] + s] * pd + c + base;
+          } else {
+            for (int64_t s = 0; s < R; ++s)
+              for (int d = 0; d < pd; ++d)
+                rowval[pc + s * pd + d] = (int64_t)adj[ap[j] + s] * pd + d + base;
+          }
         }
       }
     }
+  };
+  const int64_t nt = N < 65536 ? 1 : std::max<int64_t>(1, std::min<int64_t>(8, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t j0 = N * t / nt, j1 = N * (t + 1) / nt;
+    if (t + 1 == nt) fill(j0, j1); else th.emplace_back(fill, j0, j1);
   }
+  for (auto& x : th) x.join();
   if (colptr) colptr[N * pd] = *nnz + base;
   return LLFEM_OK;
 }
