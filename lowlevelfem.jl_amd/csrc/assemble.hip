@@ -247,6 +247,25 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const doubl
   const size_t geo_doubles = (size_t)kGeoStride * m->E * (affine ? 1 : q);
   LLFEM_TRY(ensure_buf(ctx, &m->d_geo, &m->geo_bytes, sizeof(double) * geo_doubles));
   const int64_t threads = affine ? m->E : m->E * q;
+  if (affine) {
+    const unsigned g = (unsigned)((m->E + 255) / 256);
+    const double* dN0 = m->d_tab;   // gradients at the first point
+#define LLFEM_GEO_AFFINE(n)                                                                        \
+  case n:                                                                                          \
+    k_geometry_affine<n><<<g, 256, 0, ctx->stream>>>(m->E, m->N, m->d_xyz, m->d_conn, dN0, m->d_geo, \
+                                                      d_scale);                                    \
+    return LLFEM_OK;
+    switch (npe) {
+      LLFEM_GEO_AFFINE(4)
+      LLFEM_GEO_AFFINE(6)
+      LLFEM_GEO_AFFINE(8)
+      LLFEM_GEO_AFFINE(10)
+      LLFEM_GEO_AFFINE(20)
+      LLFEM_GEO_AFFINE(27)
+      default: break;
+    }
+#undef LLFEM_GEO_AFFINE
+  }
   k_geometry<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,
       m->d_tab + (size_t)q * npe * 3, m->d_geo, d_scale);

@@ -121,6 +121,42 @@ __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int 
   }
 }
 
+// Affine elements with a compile-time node count: the same sums in the same
+// order as jacobian(), with every connectivity and coordinate load issued
+// before the first is used (the run-time loop waits for each node's gathers
+// in turn).
+template <int NPE>
+__global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, int64_t N, const double* __restrict__ xyz,
+                                                         const int32_t* __restrict__ conn,
+                                                         const double* __restrict__ dN, double* __restrict__ geo,
+                                                         const double* __restrict__ scale) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int32_t v[NPE];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) v[a] = conn[(int64_t)a * E + e];
+  double x[NPE], y[NPE], z[NPE];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    x[a] = xyz[v[a]];
+    y[a] = xyz[N + v[a]];
+    z[a] = xyz[2 * N + v[a]];
+  }
+  double J[9], Mi[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+#pragma unroll
+  for (int a = 1; a < NPE; ++a) {
+    const double dx = x[a] - x[0], dy = y[a] - y[0], dz = z[a] - z[0];
+    const double g0 = dN[3 * a], g1 = dN[3 * a + 1], g2 = dN[3 * a + 2];
+    J[0] += dx * g0; J[1] += dx * g1; J[2] += dx * g2;
+    J[3] += dy * g0; J[4] += dy * g1; J[5] += dy * g2;
+    J[6] += dz * g0; J[7] += dz * g1; J[8] += dz * g2;
+  }
+  const double det = inv3(J, Mi);
+  geometry_store(geo + (size_t)e * kGeoStride, Mi, fabs(det) * (scale ? scale[e] : 1.0));
+}
+
 // Affine: thread per element.  Curved: thread per (element, point), so
 // high-order elements with many points keep the device busy.
 __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
