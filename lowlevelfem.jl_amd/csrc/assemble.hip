@@ -205,6 +205,14 @@ std::vector<double> operator_table(const llfem_op_desc* d, int npe, bool affine)
   return tab;
 }
 
+// Offset (doubles) in m->d_tab of the point-fastest gradient table: after
+// [dN | w | op table (double) | op table (float)].
+size_t dnt_offset(int npe, int q) {
+  const size_t geo_tab = (size_t)q * npe * 3 + q;
+  const size_t op_max = std::max<size_t>((size_t)npe * npe * 9, (size_t)q * npe * 3);
+  return geo_tab + op_max + op_max / 2 + 2;
+}
+
 // Upload [dN | w | op-table space] and run the geometry check once per rule.
 int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
   llfem_ctx* ctx = m->ctx;
@@ -213,10 +221,8 @@ int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
   uint64_t key = hash_bytes(d->dN_ref, sizeof(double) * (size_t)q * npe * 3);
   key = hash_bytes(d->weights, sizeof(double) * (size_t)q, key);
   RuleState& rs = m->rules[key];
-  const size_t geo_tab = (size_t)q * npe * 3 + q;
-  const size_t op_max = std::max<size_t>((size_t)npe * npe * 9, (size_t)q * npe * 3);
-  // [dN | w | op table (double) | op table (float)]
-  LLFEM_TRY(ensure_buf(ctx, &m->d_tab, &m->tab_bytes, sizeof(double) * (geo_tab + op_max + op_max / 2 + 2)));
+  // [dN | w | op table (double) | op table (float) | dN points fastest]
+  LLFEM_TRY(ensure_buf(ctx, &m->d_tab, &m->tab_bytes, sizeof(double) * (dnt_offset(npe, q) + (size_t)q * npe * 3)));
   LLFEM_HIP(hipMemcpyAsync(m->d_tab, d->dN_ref, sizeof(double) * (size_t)q * npe * 3,
                            hipMemcpyHostToDevice, st));
   LLFEM_HIP(hipMemcpyAsync(m->d_tab + (size_t)q * npe * 3, d->weights, sizeof(double) * q,
@@ -265,6 +271,14 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const doubl
       default: break;
     }
 #undef LLFEM_GEO_AFFINE
+  }
+  if (!affine) {
+    double* dNT = m->d_tab + dnt_offset(npe, q);
+    const int64_t nt = (int64_t)q * npe * 3;
+    k_dn_transpose<<<(unsigned)((nt + 255) / 256), 256, 0, ctx->stream>>>(npe, q, m->d_tab, dNT);
+    k_geometry_curved<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
+        m->E, m->N, npe, q, m->d_xyz, m->d_conn, dNT, m->d_tab + (size_t)q * npe * 3, m->d_geo, d_scale);
+    return LLFEM_OK;
   }
   k_geometry<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,

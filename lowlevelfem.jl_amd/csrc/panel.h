@@ -157,6 +157,51 @@ __global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, int64_t N, c
   geometry_store(geo + (size_t)e * kGeoStride, Mi, fabs(det) * (scale ? scale[e] : 1.0));
 }
 
+// dNT[(a * q + p) * 3 + c] = dN[(p * npe + a) * 3 + c]: the gradient table
+// with the points fastest.
+__global__ __launch_bounds__(256) void k_dn_transpose(int npe, int q, const double* __restrict__ dN,
+                                                      double* __restrict__ dNT) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)q * npe * 3) return;
+  const int c = (int)(t % 3);
+  const int64_t r = t / 3;
+  const int p = (int)(r % q);
+  const int64_t a = r / q;
+  dNT[t] = dN[((int64_t)p * npe + a) * 3 + c];
+}
+
+// Curved elements: thread per (element, point), points fastest, the same sums
+// in the same order as jacobian(); the gradients come from dNT, so the lanes
+// of a wave (neighbouring points) load neighbouring doubles instead of rows
+// npe * 24 bytes apart.
+__global__ __launch_bounds__(256) void k_geometry_curved(int64_t E, int64_t N, int npe, int q,
+                                                         const double* __restrict__ xyz,
+                                                         const int32_t* __restrict__ conn,
+                                                         const double* __restrict__ dNT,
+                                                         const double* __restrict__ w, double* __restrict__ geo,
+                                                         const double* __restrict__ scale) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * q) return;
+  const int64_t e = t / q;
+  const int p = (int)(t - e * q);
+  double J[9], Mi[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+  const int32_t v0 = conn[e];
+  const double x0 = xyz[v0], y0 = xyz[N + v0], z0 = xyz[2 * N + v0];
+  for (int a = 1; a < npe; ++a) {
+    const int32_t v = conn[(int64_t)a * E + e];
+    const double x = xyz[v] - x0, y = xyz[N + v] - y0, z = xyz[2 * N + v] - z0;
+    const double* g = dNT + ((int64_t)a * q + p) * 3;
+    const double g0 = g[0], g1 = g[1], g2 = g[2];
+    J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
+    J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
+    J[6] += z * g0; J[7] += z * g1; J[8] += z * g2;
+  }
+  const double det = inv3(J, Mi);
+  geometry_store(geo + (size_t)t * kGeoStride, Mi, fabs(det) * w[p] * (scale ? scale[e] : 1.0));
+}
+
 // Affine: thread per element.  Curved: thread per (element, point), so
 // high-order elements with many points keep the device busy.
 __global__ void k_geometry(int64_t E, int64_t N, int npe, int q, int affine,
