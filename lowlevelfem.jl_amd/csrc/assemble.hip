@@ -27,6 +27,7 @@
 //   * K_ba[d][c] = lam G[d][c] + mu G[c][d] + mu tr(G) delta_dc for Hooke's
 //     law, or sum C[d][al][c][be] G[al][be] for a general 6x6 D.
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -395,7 +396,38 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
                                  iso ? nullptr : P.C, target));
   else
     LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
-  for (const PanelLaunch& L : (highorder ? std::vector<PanelLaunch>() : m->launches)) {
+  // Node classes write disjoint panels, so their launches may overlap: each
+  // launch goes to the least-loaded of up to three streams (largest work
+  // first; work = steps of the launch), which fills the tail of one class's
+  // launch with the next.  LLFEM_PANEL_STREAMS=n (1..3) sets the count.
+  const std::vector<PanelLaunch>& plan = highorder ? std::vector<PanelLaunch>() : m->launches;
+  int nstreams = 3;
+  if (const char* env = std::getenv("LLFEM_PANEL_STREAMS")) nstreams = std::max(1, std::min(3, std::atoi(env)));
+  nstreams = std::min<int>(nstreams, (int)plan.size());
+  std::vector<int> stream_of(plan.size(), 0);
+  if (nstreams > 1) {
+    std::vector<size_t> by_work(plan.size());
+    std::iota(by_work.begin(), by_work.end(), size_t(0));
+    auto work = [&](size_t i) { return plan[i].waves * (int64_t)plan[i].rounds * plan[i].cnt; };
+    std::stable_sort(by_work.begin(), by_work.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+    std::vector<int64_t> load(nstreams, 0);
+    for (size_t i : by_work) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      stream_of[i] = k;
+      load[k] += work(i);
+    }
+    if (!ctx->fork_ev) LLFEM_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    LLFEM_HIP(hipEventRecord(ctx->fork_ev, st));   // after the geometry
+    for (int k = 0; k + 1 < nstreams; ++k) {
+      if (!ctx->side[k]) LLFEM_HIP(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+      if (!ctx->join_ev[k]) LLFEM_HIP(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+      LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+    }
+  }
+  int lerr = LLFEM_OK;
+  for (size_t li = 0; li < plan.size(); ++li) {
+    const PanelLaunch& L = plan[li];
+    hipStream_t ls = stream_of[li] == 0 ? st : ctx->side[stream_of[li] - 1];
     P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
     P.desc = m->d_desc + L.desc_base;
     P.waves = L.waves;
@@ -415,7 +447,10 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
       const size_t resident = c * (kLdsPerCU / (c * wave_bytes));
       if (resident > best) { best = resident; wpb = (int)c; }
     }
-    if (wpb < 1) return fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
+    if (wpb < 1) {   // joined below before the error is returned
+      lerr = fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
+      break;
+    }
     const auto tu = ctx->tune.find(m->etype);
     if (tu != ctx->tune.end() && tu->second.wpb > 0 && (size_t)tu->second.wpb <= max_wpb &&
         (size_t)tu->second.wpb * wave_bytes <= kLdsPerCU)
@@ -432,9 +467,14 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     if (lds_bytes > 65536)
       LLFEM_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds_bytes));
-    hipLaunchKernelGGL(lfn, dim3(blocks), dim3(64 * wpb), lds_bytes, st, P);
+    hipLaunchKernelGGL(lfn, dim3(blocks), dim3(64 * wpb), lds_bytes, ls, P);
   }
   LLFEM_HIP(hipGetLastError());
+  for (int k = 0; k + 1 < nstreams; ++k) {   // join
+    LLFEM_HIP(hipEventRecord(ctx->join_ev[k], ctx->side[k]));
+    LLFEM_HIP(hipStreamWaitEvent(st, ctx->join_ev[k], 0));
+  }
+  LLFEM_TRY(lerr);
   if (!highorder) LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
   if (op == LLFEM_OP_MASS && d->lumped) {
     if (!m->d_lumped) LLFEM_TRY(dalloc(ctx, &m->d_lumped, 3 * m->N));

@@ -93,12 +93,17 @@ struct llfem_ctx {
   size_t budget = 0;
   size_t in_use = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // panel launches of one assembly may run on the side streams too (default
+  // priority, created on first use), forked from and joined to `stream`
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[2] = {nullptr, nullptr};
   int num_cus = 0;
   std::map<int, llfem::Tune> tune;                 // by gmsh element type
   std::map<uint64_t, llfem::DevTable> tables;      // by content hash
   void* d_scratch = nullptr;                       // batch descriptors + queue counter
   size_t scratch_bytes = 0;
-  double* d_sink = nullptr;                        // 128 doubles: target of padding-lane panel stores
+  double* d_sink = nullptr;                        // 64 doubles: target of padding-lane panel stores
   // pinned staging for large device-to-host copies (copy_to_host)
   void* h_stage[2] = {nullptr, nullptr};
   size_t stage_bytes = 0;

@@ -152,7 +152,15 @@ int llfem_ctx_destroy(llfem_ctx* ctx) {
   for (auto& kv : ctx->tables) llfem::dfree(ctx, kv.second.ptr, kv.second.bytes);
   ctx->tables.clear();
   if (ctx->d_scratch) llfem::dfree(ctx, ctx->d_scratch, ctx->scratch_bytes);
-  if (ctx->d_sink) llfem::dfree(ctx, ctx->d_sink, 128 * sizeof(double));
+  if (ctx->d_sink) llfem::dfree(ctx, ctx->d_sink, 64 * sizeof(double));
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->side[k]) {
+      (void)hipStreamSynchronize(ctx->side[k]);
+      (void)hipStreamDestroy(ctx->side[k]);
+    }
+    if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (int b = 0; b < 2; ++b) {
     if (ctx->h_stage[b]) (void)hipHostFree(ctx->h_stage[b]);
     if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
