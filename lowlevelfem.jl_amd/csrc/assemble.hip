@@ -416,13 +416,8 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
       stream_of[i] = k;
       load[k] += work(i);
     }
-    if (!ctx->fork_ev) LLFEM_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     LLFEM_HIP(hipEventRecord(ctx->fork_ev, st));   // after the geometry
-    for (int k = 0; k + 1 < nstreams; ++k) {
-      if (!ctx->side[k]) LLFEM_HIP(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
-      if (!ctx->join_ev[k]) LLFEM_HIP(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
-      LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
-    }
+    for (int k = 0; k + 1 < nstreams; ++k) LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
   }
   int lerr = LLFEM_OK;
   for (size_t li = 0; li < plan.size(); ++li) {

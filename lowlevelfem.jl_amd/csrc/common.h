@@ -94,7 +94,7 @@ struct llfem_ctx {
   size_t in_use = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // panel launches of one assembly may run on the side streams too (default
-  // priority, created on first use), forked from and joined to `stream`
+  // priority), forked from and joined to `stream`
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev[2] = {nullptr, nullptr};

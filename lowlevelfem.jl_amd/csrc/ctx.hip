@@ -139,6 +139,19 @@ int llfem_ctx_create(int device, size_t hbm_budget_bytes, llfem_ctx** out) {
     return fail(LLFEM_EHIP, "hipStreamCreate failed");
   }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  // side streams for concurrent panel launches, created here so the first
+  // assembly does not pay for them
+  for (int k = 0; k < 2; ++k) {
+    if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) != hipSuccess) {
+      llfem_ctx_destroy(c);
+      return fail(LLFEM_EHIP, "hipStreamCreate failed");
+    }
+  }
+  if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess) {
+    llfem_ctx_destroy(c);
+    return fail(LLFEM_EHIP, "hipEventCreate failed");
+  }
   *out = c;
   return LLFEM_OK;
 }
