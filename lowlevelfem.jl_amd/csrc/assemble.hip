@@ -459,9 +459,14 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
       if (!std::getenv("LLFEM_PANEL_NO_ROUNDS"))        // A/B knob: the step-unrolled kernel
         if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) lfn = f;
     }
-    if (lds_bytes > 65536)
-      LLFEM_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_bytes));
+    if (lds_bytes > 65536) {
+      const hipError_t e = hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds_bytes);
+      if (e != hipSuccess) {   // joined below before the error is returned
+        lerr = fail(LLFEM_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+        break;
+      }
+    }
     hipLaunchKernelGGL(lfn, dim3(blocks), dim3(64 * wpb), lds_bytes, ls, P);
   }
   LLFEM_HIP(hipGetLastError());
