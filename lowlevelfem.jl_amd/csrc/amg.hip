@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <future>
@@ -840,6 +842,18 @@ int amg_gather(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgInput*
   in->strong.resize((size_t)nadj);
   in->xyz.resize((size_t)3 * N);
   in->mask.resize((size_t)3 * N);
+  // first touch of the uninitialised block arrays on all host threads, while
+  // the strength kernel runs: the copies would otherwise take their page
+  // faults one 4 KB page at a time in the runtime's copy loop
+  auto touch = [](auto& v) {
+    uint8_t* p = reinterpret_cast<uint8_t*>(v.data());
+    const int64_t bytes = (int64_t)(v.size() * sizeof(v[0]));
+    parallel_for((bytes + 4095) / 4096, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) p[i * 4096] = 0;
+    });
+  };
+  touch(in->adj);
+  touch(in->strong);
   hipError_t e = hipMemcpyAsync(in->adj_ptr.data(), sp.adj_ptr, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipMemcpyAsync(in->adj.data(), sp.adj, sizeof(int32_t) * nadj, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipMemcpyAsync(in->dslot.data(), sp.dslot, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
@@ -1418,11 +1432,23 @@ void amg_batch_free(llfem_ctx* ctx, AmgBatch* b) {
 const std::vector<double>& amg_lambda(const AmgSys* s) { return s->lambda; }
 
 int amg_build(llfem_ctx* ctx, const SysSpec& sp, const double* d_xyz, AmgHier** out) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   AmgInput in;
   LLFEM_TRY(amg_gather(ctx, sp, d_xyz, &in));
+  const auto t1 = clk::now();
   AmgHost H;
   LLFEM_TRY(amg_build_host(in, &H));
-  return amg_upload(ctx, H, sp, out);
+  const auto t2 = clk::now();
+  const int rc = amg_upload(ctx, H, sp, out);
+  if (std::getenv("LLFEM_AMG_TRACE")) {   // set-up phases, wall clock
+    (void)hipStreamSynchronize(ctx->stream);
+    const auto t3 = clk::now();
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    std::fprintf(stderr, "[amg] N %lld: gather %.1f ms, host build %.1f ms, upload %.1f ms\n", (long long)sp.N,
+                 ms(t0, t1), ms(t1, t2), ms(t2, t3));
+  }
+  return rc;
 }
 
 // Asynchronous build: the device gather runs at once, the host build on a
