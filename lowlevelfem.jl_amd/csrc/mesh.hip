@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <numeric>
 
+#include <memory>
 #include <thread>
 
 #include "common.h"
@@ -526,10 +527,26 @@ int llfem_mesh_pattern(llfem_mesh* m, int32_t op, int32_t one_based, int64_t* nn
   *nnz = m->nadj * blk;
   if (!colptr && !rowval) return LLFEM_OK;
   const int64_t base = one_based ? 1 : 0;
-  std::vector<int32_t> adj;
+  const int64_t nt = N < 65536 ? 1 : std::max<int64_t>(1, std::min<int64_t>(8, std::thread::hardware_concurrency()));
+  auto on_threads = [&](int64_t n, auto&& f) {   // f(lo, hi) over contiguous ranges of [0, n)
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t) {
+      const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+      if (t + 1 == nt) f(lo, hi); else th.emplace_back(f, lo, hi);
+    }
+    for (auto& x : th) x.join();
+  };
+  std::unique_ptr<int32_t[]> adj;   // left uninitialised: the copy overwrites it
   if (rowval) {
-    adj.resize(m->nadj);
-    LLFEM_HIP(hipMemcpy(adj.data(), m->d_adj, sizeof(int32_t) * m->nadj, hipMemcpyDeviceToHost));
+    adj.reset(new int32_t[m->nadj]);
+    // first touch on the host threads (the runtime's copy would otherwise
+    // fault the pages in one at a time)
+    uint8_t* p = reinterpret_cast<uint8_t*>(adj.get());
+    const int64_t pages = (int64_t)((sizeof(int32_t) * m->nadj + 4095) / 4096);
+    on_threads(pages, [p](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) p[i * 4096] = 0;
+    });
+    LLFEM_HIP(hipMemcpy(adj.get(), m->d_adj, sizeof(int32_t) * m->nadj, hipMemcpyDeviceToHost));
   }
   const auto& ap = m->h_adj_ptr;
   // columns of disjoint node ranges fill disjoint parts of colptr / rowval:
@@ -553,13 +570,7 @@ int llfem_mesh_pattern(llfem_mesh* m, int32_t op, int32_t one_based, int64_t* nn
       }
     }
   };
-  const int64_t nt = N < 65536 ? 1 : std::max<int64_t>(1, std::min<int64_t>(8, std::thread::hardware_concurrency()));
-  std::vector<std::thread> th;
-  for (int64_t t = 0; t < nt; ++t) {
-    const int64_t j0 = N * t / nt, j1 = N * (t + 1) / nt;
-    if (t + 1 == nt) fill(j0, j1); else th.emplace_back(fill, j0, j1);
-  }
-  for (auto& x : th) x.join();
+  on_threads(N, fill);
   if (colptr) colptr[N * pd] = *nnz + base;
   return LLFEM_OK;
 }
