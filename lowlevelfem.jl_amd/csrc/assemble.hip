@@ -400,7 +400,8 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   // launch goes to the least-loaded of two streams (largest work first; work
   // = steps of the launch), which fills the tail of one class's launch with
   // the next.  LLFEM_PANEL_STREAMS=n (1..3) sets the count.
-  const std::vector<PanelLaunch>& plan = highorder ? std::vector<PanelLaunch>() : m->launches;
+  static const std::vector<PanelLaunch> kNoLaunches;
+  const std::vector<PanelLaunch>& plan = highorder ? kNoLaunches : m->launches;
   int nstreams = 2;
   if (const char* env = std::getenv("LLFEM_PANEL_STREAMS")) nstreams = std::max(1, std::min(3, std::atoi(env)));
   nstreams = std::min<int>(nstreams, (int)plan.size());
