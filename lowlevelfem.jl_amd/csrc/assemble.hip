@@ -102,27 +102,39 @@ PanelFn select_fixed_stores(int npe, int map_bytes, int op, bool affine, bool is
 // structured tet10 and hex8 meshes: (npe, steps per node, ring, stores per
 // panel).  A launch whose (npe, cnt) is listed and whose panels fit the
 // store count runs it; every other launch keeps k_panel.
-template <typename Real>
+template <typename Real, bool TL>
 PanelFn select_rounds_real(int npe, int cnt, int ks_needed) {
   struct V {
     int npe, cnt, ks;
     PanelFn fn;
   };
   static const V table[] = {
-      {10, 4, 4, k_panel_rounds<10, Real, 4, 2, 4>},   {10, 4, 6, k_panel_rounds<10, Real, 4, 2, 6>},
-      {10, 6, 4, k_panel_rounds<10, Real, 6, 3, 4>},   {10, 6, 6, k_panel_rounds<10, Real, 6, 3, 6>},
-      {10, 24, 10, k_panel_rounds<10, Real, 24, 3, 10>}, {10, 24, 12, k_panel_rounds<10, Real, 24, 3, 12>},
-      {8, 8, 4, k_panel_rounds<8, Real, 8, 2, 4>},     {8, 8, 6, k_panel_rounds<8, Real, 8, 2, 6>},
+      {10, 4, 4, k_panel_rounds<10, Real, 4, 2, 4, TL>},   {10, 4, 6, k_panel_rounds<10, Real, 4, 2, 6, TL>},
+      {10, 6, 4, k_panel_rounds<10, Real, 6, 3, 4, TL>},   {10, 6, 6, k_panel_rounds<10, Real, 6, 3, 6, TL>},
+      {10, 24, 10, k_panel_rounds<10, Real, 24, 3, 10, TL>}, {10, 24, 12, k_panel_rounds<10, Real, 24, 3, 12, TL>},
+      {8, 8, 4, k_panel_rounds<8, Real, 8, 2, 4, TL>},     {8, 8, 6, k_panel_rounds<8, Real, 8, 2, 6, TL>},
   };
   for (const V& v : table)
     if (v.npe == npe && v.cnt == cnt && v.ks >= ks_needed) return v.fn;
   return nullptr;
 }
 
+// Reference table in LDS for the FP64 round kernels: each lane reads its 9
+// table values from LDS instead of through the texture-address unit, the
+// busiest unit of these kernels (FP64 assembly of the tet10 beam 2.09 -> 2.04
+// ms; the FP32 kernels measured 2 % slower with it and keep the global table).
+// LLFEM_PANEL_TABLE_LDS=0 switches it off for A/B runs.
+bool rounds_table_lds() {
+  const char* e = std::getenv("LLFEM_PANEL_TABLE_LDS");
+  return !e || std::atoi(e) != 0;
+}
+
 PanelFn select_rounds(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32, int cnt, int max_r) {
   if (map_bytes != 1 || op != LLFEM_OP_STIFFNESS || !affine || !iso) return nullptr;
   const int need = (9 * max_r + 7 + 63) / 64;   // up to 7 doubles of window alignment
-  return fp32 ? select_rounds_real<float>(npe, cnt, need) : select_rounds_real<double>(npe, cnt, need);
+  if (fp32) return select_rounds_real<float, false>(npe, cnt, need);
+  return rounds_table_lds() ? select_rounds_real<double, true>(npe, cnt, need)
+                            : select_rounds_real<double, false>(npe, cnt, need);
 }
 
 PanelFn select_panel(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32) {
@@ -435,12 +447,24 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     size_t max_wpb = (iso || op == LLFEM_OP_MASS) ? 8 : 4;   // matches k_panel launch bounds
     if (const char* env = std::getenv("LLFEM_PANEL_MAX_WAVES"))     // tuning knob
       max_wpb = std::max<size_t>(1, std::min<size_t>(max_wpb, (size_t)std::atoi(env)));
+    // kernel of the launch; a round kernel may keep the reference table in LDS
+    // after the block's panels (block_extra bytes, 16-byte aligned start)
+    PanelFn lfn = fn;
+    size_t block_extra = 0;
+    if (!std::getenv("LLFEM_PANEL_RUNTIME_STORES")) {   // A/B knob: the run-time store loop
+      if (PanelFn f = select_fixed_stores(npe, m->map_bytes, op, affine, iso, fp32, L.max_r)) lfn = f;
+      if (!std::getenv("LLFEM_PANEL_NO_ROUNDS"))        // A/B knob: the step-unrolled kernel
+        if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) {
+          lfn = f;
+          if (rounds_table_lds() && !fp32) block_extra = 8 + 10 * sizeof(double) * npe * npe;
+        }
+    }
     // waves per block: the size that packs the most waves into one CU's LDS
     // (several small blocks beat one large block with LDS left over)
     int wpb = 0;
     size_t best = 0;
-    for (size_t c = 1; c <= max_wpb && c * wave_bytes <= kLdsPerCU; ++c) {
-      const size_t resident = c * (kLdsPerCU / (c * wave_bytes));
+    for (size_t c = 1; c <= max_wpb && c * wave_bytes + block_extra <= kLdsPerCU; ++c) {
+      const size_t resident = c * (kLdsPerCU / (c * wave_bytes + block_extra));
       if (resident > best) { best = resident; wpb = (int)c; }
     }
     if (wpb < 1) {   // joined below before the error is returned
@@ -449,17 +473,11 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     }
     const auto tu = ctx->tune.find(m->etype);
     if (tu != ctx->tune.end() && tu->second.wpb > 0 && (size_t)tu->second.wpb <= max_wpb &&
-        (size_t)tu->second.wpb * wave_bytes <= kLdsPerCU)
+        (size_t)tu->second.wpb * wave_bytes + block_extra <= kLdsPerCU)
       wpb = tu->second.wpb;
     const int64_t used = (L.waves + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
-    const size_t lds_bytes = wave_bytes * wpb;
-    PanelFn lfn = fn;
-    if (!std::getenv("LLFEM_PANEL_RUNTIME_STORES")) {   // A/B knob: the run-time store loop
-      if (PanelFn f = select_fixed_stores(npe, m->map_bytes, op, affine, iso, fp32, L.max_r)) lfn = f;
-      if (!std::getenv("LLFEM_PANEL_NO_ROUNDS"))        // A/B knob: the step-unrolled kernel
-        if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) lfn = f;
-    }
+    const size_t lds_bytes = wave_bytes * wpb + block_extra;
     if (lds_bytes > 65536) {
       const hipError_t e = hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)lds_bytes);

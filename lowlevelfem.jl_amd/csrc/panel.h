@@ -291,7 +291,8 @@ __device__ __forceinline__ void load9(const double* d, const float* f, Real* o) 
   }
 }
 
-// G = Minv^T Gh Minv for one block (affine elements).
+// G = Minv^T Gh Minv for one block (affine elements), as explicit fused
+// multiply-adds (the same rounding in every kernel that uses it).
 template <typename Real>
 __device__ __forceinline__ void block_G_affine(const Geo<Real>& g, const Real* Gh, Real* G) {
   const Real* Mi = g.Mi;
@@ -300,27 +301,37 @@ __device__ __forceinline__ void block_G_affine(const Geo<Real>& g, const Real* G
   for (int al = 0; al < 3; ++al)
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk)
-      T[al * 3 + kk] = Gh[al * 3] * Mi[kk] + Gh[al * 3 + 1] * Mi[3 + kk] + Gh[al * 3 + 2] * Mi[6 + kk];
+      T[al * 3 + kk] =
+          __builtin_fma(Gh[al * 3 + 2], Mi[6 + kk], __builtin_fma(Gh[al * 3 + 1], Mi[3 + kk], Gh[al * 3] * Mi[kk]));
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk)
-      G[i * 3 + kk] = Mi[i] * T[kk] + Mi[3 + i] * T[3 + kk] + Mi[6 + i] * T[6 + kk];
+      G[i * 3 + kk] = __builtin_fma(Mi[6 + i], T[6 + kk], __builtin_fma(Mi[3 + i], T[3 + kk], Mi[i] * T[kk]));
+}
+
+// Hooke's law on one block: K[d][c] = lam G[d][c] + mu G[c][d] + mu tr(G)
+// delta_dc, all scaled by `scale`.  The products are written as explicit
+// fused multiply-adds so every kernel that shares this function rounds the
+// same way whatever the compiler's contraction choices around it.
+template <typename Real>
+__device__ __forceinline__ void hooke_block(Real lam, Real mu, const Real* G, Real scale, Real* K) {
+  const Real l = lam * scale, u = mu * scale;
+  const Real tr = u * (G[0] + G[4] + G[8]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) K[d * 3 + c] = __builtin_fma(l, G[d * 3 + c], u * G[c * 3 + d]);
+  K[0] += tr;
+  K[4] += tr;
+  K[8] += tr;
 }
 
 // K[d][c] = Ke[(b,d),(a,c)] from G and the material.
 template <bool ISO, typename Real>
 __device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Real scale, Real* K) {
   if constexpr (ISO) {
-    const Real l = (Real)P.lam * scale, u = (Real)P.mu * scale;
-    const Real tr = u * (G[0] + G[4] + G[8]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) K[d * 3 + c] = l * G[d * 3 + c] + u * G[c * 3 + d];
-    K[0] += tr;
-    K[4] += tr;
-    K[8] += tr;
+    hooke_block<Real>((Real)P.lam, (Real)P.mu, G, scale, K);
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -674,8 +685,14 @@ __device__ __forceinline__ void panel_wave(const PanelParams& P, int64_t w0, int
 // counts are exact everywhere: after the stores, the first PFX steps consume
 // loads issued before them without waiting for the stores, and the stores
 // have PFX steps to complete before the next wait that covers them.
-template <int NPE, typename Real, int CNT, int PFX, int KS>
-__device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t w0, double* wl) {
+// TL: the reference table sits in LDS (tl, entries of kTabLds<Real> values)
+// instead of being fetched from global memory with the geometry.
+template <typename Real>
+constexpr int kTabLds = sizeof(Real) == 8 ? 10 : 12;   // 16-byte aligned entries
+
+template <int NPE, typename Real, int CNT, int PFX, int KS, bool TL = false>
+__device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t w0, double* wl,
+                                                  const Real* tl = nullptr) {
   static_assert(CNT % (2 * PFX) == 0, "the ring must wrap at round ends");
   constexpr int NPW = 64 / NPE;
   constexpr int OP = LLFEM_OP_STIFFNESS;
@@ -704,6 +721,17 @@ __device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t 
   int2 dcur = desc[lane_rec];
   int2 dnext = desc[(G > 1 ? NPW : 0) + lane_rec];
 
+  auto tab_at = [&](int a) -> TabBlk<Real> {
+    if constexpr (TL) {
+      TabBlk<Real> t;
+      const Real* e = tl + (a * NPE + b) * kTabLds<Real>;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) t.v[k] = e[k];
+      return t;
+    } else {
+      return load_tab<NPE, OP, true, Real>(tab, tabf, nullptr, a, b);
+    }
+  };
   RawRec rec[2 * PFX];
   Geo<Real> g[PFX];
   TabBlk<Real> tb[PFX];
@@ -713,7 +741,7 @@ __device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t 
   for (int v = 0; v < PFX; ++v) {
     const int a = raw_loc(rec[v]);
     g[v] = load_geo<Real>(geo + (size_t)raw_elem(rec[v]) * kGeoStride);
-    tb[v] = load_tab<NPE, OP, true, Real>(tab, tabf, nullptr, a < 0 ? 0 : a, b);
+    tb[v] = tab_at(a < 0 ? 0 : a);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -733,7 +761,7 @@ __device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t 
       const RawRec& nx = rec[(u + PFX) % (2 * PFX)];
       const int na = raw_loc(nx);
       g[u % PFX] = load_geo<Real>(geo + (size_t)raw_elem(nx) * kGeoStride);
-      tb[u % PFX] = load_tab<NPE, OP, true, Real>(tab, tabf, nullptr, na < 0 ? 0 : na, b);
+      tb[u % PFX] = tab_at(na < 0 ? 0 : na);
       rec[u % (2 * PFX)] = load_raw(recs, min(rbase + (s + 2 * PFX) * NPW + lane_rec, rlast));
       // keep each step's loads in their step: the scheduler would otherwise
       // sink them towards their uses and shorten the prefetch distance
@@ -773,16 +801,33 @@ __device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t 
   }
 }
 
-template <int NPE, typename Real, int CNT, int PFX, int KS>
+template <int NPE, typename Real, int CNT, int PFX, int KS, bool TL = false>
 __global__ __launch_bounds__(512) void k_panel_rounds(PanelParams P) {
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6;
+  Real* tl = nullptr;
+  if constexpr (TL) {   // the table after the waves' panels (16-byte aligned); every wave reaches the barrier
+    tl = reinterpret_cast<Real*>(lds + (((size_t)(blockDim.x >> 6) * P.lds_per_wave + 1) & ~(size_t)1));
+    const double* tab = (const double*)gp(P.tab);
+    const float* tabf = (const float*)gp(P.tabf);
+    constexpr int TS = kTabLds<Real>;
+    for (int i = threadIdx.x; i < NPE * NPE * TS; i += blockDim.x) {
+      const int ent = i / TS, k = i - ent * TS;
+      Real v = 0;
+      if (k < 9) {
+        if constexpr (sizeof(Real) == 8) v = tab[ent * 9 + k];
+        else v = tabf[ent * 9 + k];
+      }
+      tl[i] = v;
+    }
+    __syncthreads();
+  }
   // XCD-aware work order, as in k_panel
   const int64_t per = gridDim.x / kXcds;
   const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   const int64_t w = lb * (blockDim.x >> 6) + wave;
   if (w >= P.waves) return;
-  panel_wave_rounds<NPE, Real, CNT, PFX, KS>(P, w, lds + (size_t)wave * P.lds_per_wave);
+  panel_wave_rounds<NPE, Real, CNT, PFX, KS, TL>(P, w, lds + (size_t)wave * P.lds_per_wave, tl);
 }
 
 template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real, int KS = 0>
