@@ -91,6 +91,14 @@ typedef struct {
                               the results added in group order -- the order in which
                               the reference's sparse() combines the groups' triplets.
                               llfem_assemble only (batch and server: NULL). */
+  int32_t accumulate;      /* llfem_assemble: 0 replaces the mesh's resident values of
+                              this operator, 1 adds this call's values to them (the
+                              first material group with 0, the others with 1: the mesh
+                              then holds the sum, in group order).  A resident operator
+                              assembled with element_scale counts as complete -- usable
+                              by llfem_solve / llfem_mesh_values -- once every element
+                              has had a non-zero factor in some call since the last
+                              accumulate = 0 call.  Batch and server: 0. */
 } llfem_op_desc;
 
 typedef struct {
@@ -106,6 +114,12 @@ typedef struct {
   double alg_flops;        /* element-integration flops of the call (see llfem_assemble) */
   double setup_ms;         /* solver: preconditioner set-up wall time (AMG: hierarchy build when
                               not cached, plus the per-solve coarse operators) */
+  int64_t rollbacks;       /* solver (reliable-update methods): updates whose FP32 correction
+                              was not finite, so the previous FP64 solution was kept and the
+                              recursion restarted from it */
+  int64_t restarts;        /* solver (AMG): reliable updates whose preconditioned residual
+                              was recomputed by a V-cycle because the rescaled one was no
+                              longer consistent (r.z <= 0 or relative drift > 1/4) */
 } llfem_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -292,6 +306,8 @@ typedef struct {
   double relres;              /* ||(K u - f)_free|| / ||f_free|| */
   int32_t mesh_cache_hit;
   int32_t status;             /* LLFEM_OK or the request's error code */
+  int64_t rollbacks;          /* as llfem_stats.rollbacks */
+  int64_t restarts;           /* as llfem_stats.restarts */
 } llfem_request_stats;
 
 LLFEM_API int llfem_server_create(int device, size_t hbm_budget_bytes, int32_t max_active, llfem_server** out);

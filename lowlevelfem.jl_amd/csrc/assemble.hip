@@ -157,6 +157,11 @@ PanelFn select_panel(int npe, int map_bytes, int op, bool affine, bool iso, bool
   }
 }
 
+__global__ void k_add_into(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
 int ensure_buf(llfem_ctx* ctx, double** p, size_t* cur, size_t bytes) {
   if (*p && *cur >= bytes) return LLFEM_OK;
   if (*p) dfree(ctx, *p, *cur);
@@ -330,6 +335,14 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     if (const char* env = std::getenv("LLFEM_HIGHORDER")) highorder = highorder || std::atoi(env) != 0;
   if (d->precision == LLFEM_BF16X3 && !highorder)
     return fail(LLFEM_EUNSUP, "llfem_assemble: BF16x3 is for order >= 3 hexahedra (stiffness)");
+  const bool acc = d->accumulate != 0;
+  if (acc) {
+    if (dev_out) return fail(LLFEM_EINVAL, "llfem_assemble: accumulate is for the mesh's resident values");
+    if (d->op == LLFEM_OP_MASS && d->lumped)
+      return fail(LLFEM_EINVAL, "llfem_assemble: accumulate the consistent mass, then lump it");
+    const bool have = d->op == LLFEM_OP_STIFFNESS ? (m->have_K || m->partial[0]) : (m->have_M || m->partial[1]);
+    if (!have) return fail(LLFEM_EINVAL, "llfem_assemble: accumulate needs a resident operator to add to");
+  }
   llfem_ctx* ctx = m->ctx;
   LLFEM_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -377,6 +390,13 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     return fail(LLFEM_EINVAL, "assemble: lumped mass has no device target");
   if (!dev_out && !*outp) LLFEM_TRY(dalloc(ctx, outp, (size_t)nnz));
   double* target = dev_out ? dev_out : *outp;
+  // accumulate: this call's values go to a scratch buffer and are added to
+  // the resident ones after the panel launches
+  double* acc_tmp = nullptr;
+  if (acc) {
+    LLFEM_TRY(dalloc(ctx, &acc_tmp, (size_t)nnz));
+    target = acc_tmp;
+  }
 
   PanelParams P{};
   P.rec_bytes = m->rec_bytes;
@@ -495,6 +515,12 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   }
   LLFEM_TRY(lerr);
   if (!highorder) LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
+  if (acc) {   // resident += this call's values (group order: the earlier groups first)
+    k_add_into<<<(unsigned)((nnz + 255) / 256), 256, 0, st>>>(nnz, acc_tmp, *outp);
+    LLFEM_HIP(hipGetLastError());
+    LLFEM_HIP(hipStreamSynchronize(st));
+    drelease(ctx, acc_tmp, (size_t)nnz);
+  }
   if (op == LLFEM_OP_MASS && d->lumped) {
     if (!m->d_lumped) LLFEM_TRY(dalloc(ctx, &m->d_lumped, 3 * m->N));
     k_lump<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_valM, m->d_lumped);
@@ -502,18 +528,35 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   LLFEM_HIP(hipEventRecord(ctx->ev[3], st));
   if (dev_out) {
     // the mesh's own operator buffers are untouched
-  } else if (op == LLFEM_OP_STIFFNESS) {
-    m->have_K = true;
-    m->solver_ready = false;
   } else {
-    m->have_M = true;
-    m->have_lumped = d->lumped != 0;
+    // complete = every element contributed since the last replacing call
+    const int oi = op == LLFEM_OP_STIFFNESS ? 0 : 1;
+    bool complete = true;
+    if (d->element_scale || acc) {
+      std::vector<uint8_t>& cov = m->covered[oi];
+      if (!acc) cov.assign((size_t)m->E, 0);
+      else if (cov.empty()) cov.assign((size_t)m->E, 1);   // added to a complete operator
+      for (int64_t e = 0; e < m->E; ++e)
+        if (!d->element_scale || d->element_scale[e] != 0.0) cov[e] = 1;
+      for (int64_t e = 0; e < m->E && complete; ++e) complete = cov[e] != 0;
+      if (complete) cov.clear();
+    } else {
+      m->covered[oi].clear();
+    }
+    m->partial[oi] = !complete;
+    if (op == LLFEM_OP_STIFFNESS) {
+      m->have_K = complete;
+      m->solver_ready = false;
+    } else {
+      m->have_M = complete;
+      m->have_lumped = complete && d->lumped != 0;
+    }
   }
   if (nzval_out) {
     if (op == LLFEM_OP_MASS && d->lumped)
       LLFEM_TRY(copy_to_host(ctx, nzval_out, m->d_lumped, 3 * (size_t)m->N));
     else
-      LLFEM_TRY(copy_to_host(ctx, nzval_out, target, (size_t)nnz));
+      LLFEM_TRY(copy_to_host(ctx, nzval_out, acc ? *outp : target, (size_t)nnz));
   }
   LLFEM_HIP(hipEventSynchronize(ctx->ev[3]));
   if (stats) {
@@ -534,6 +577,9 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     stats->iterations = 0;
     stats->refinements = 0;
     stats->relres = 0;
+    stats->setup_ms = 0;
+    stats->rollbacks = 0;
+    stats->restarts = 0;
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

@@ -152,6 +152,10 @@ struct llfem_mesh {
   bool have_K = false;
   bool have_M = false;
   bool have_lumped = false;
+  // resident operators assembled per material group (element_scale): which
+  // elements have contributed since the last non-accumulating call [op]
+  bool partial[2] = {false, false};
+  std::vector<uint8_t> covered[2];
   // solver state cached per assembled K
   float* d_valK32 = nullptr;
   double* d_diagK = nullptr;
@@ -239,6 +243,7 @@ class BatchSolver {
   static constexpr double kDelta = 0.1;      // reliable-update trigger (residual reduction)
   static constexpr double kDeltaAmg = 0.1;   // the same for AMG systems
   static constexpr int kMaxRunAmg = 40;      // AMG systems: FP64 update after at most this many iterations
+  static constexpr double kDriftAmg = 0.25;  // AMG update: relative drift beyond which z is recomputed
   // f64_vectors: FP64 Krylov vectors over the FP32 operator (else FP32 vectors)
   BatchSolver(llfem_ctx* c, bool f64_vectors);
   ~BatchSolver();
@@ -300,6 +305,8 @@ void amg_job_drop(AmgJob* job);                                                 
 
 int64_t sys_iterations(const SolveSys* s);
 int64_t sys_updates(const SolveSys* s);
+int64_t sys_rollbacks(const SolveSys* s);   // updates that kept the previous solution
+int64_t sys_refreshes(const SolveSys* s);   // AMG updates that recomputed z
 int sys_status(const SolveSys* s);      // 0 running, 1 converged, -1 maxiter, -2 non-finite FP32 correction
 double sys_relres(const SolveSys* s);
 

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -391,30 +392,36 @@ __global__ __launch_bounds__(kSB) void k_ru_restart_amg(int64_t N, int32_t nvblk
 }
 
 // AMG restart without a V-cycle: r = rh * s, and p, z rescaled by pscale.
-// The true residual differs from the rescaled recursive one only by the FP32
-// drift since the last update, so z = M r carries over with the same scale.
+// z = M r carries over with the same scale only while the true residual is
+// close to the rescaled recursive one; the pass also sums the drift
+// ||r_true - pscale r_rec||^2 (into part[2 nvblk + block]) so the host can
+// tell when z must be recomputed by a V-cycle instead.
 __global__ __launch_bounds__(kSB) void k_ru_rescale_amg(int64_t N, int32_t nvblk, const double* __restrict__ rh,
                                                         double s, float pscale, float* __restrict__ r32,
                                                         float* __restrict__ z32, float* __restrict__ p32,
                                                         double* __restrict__ part) {
   __shared__ double sh[kSB];
-  double rr = 0.0, rz = 0.0;
+  double rr = 0.0, rz = 0.0, dd = 0.0;
   for (int64_t j = (int64_t)blockIdx.x * kVecNodes + threadIdx.x; j < N; j += (int64_t)nvblk * kVecNodes)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float r = (float)(rh[3 * j + c] * s);
       const float z = pscale * z32[3 * j + c];
+      const double dr = (double)r - (double)pscale * (double)r32[3 * j + c];
       r32[3 * j + c] = r;
       z32[3 * j + c] = z;
       p32[3 * j + c] = pscale * p32[3 * j + c];
       rr += (double)r * (double)r;
       rz += (double)r * (double)z;
+      dd += dr * dr;
     }
   rr = bsum(rr, sh);
   rz = bsum(rz, sh);
+  dd = bsum(dd, sh);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = rr;
     part[2 * blockIdx.x + 1] = rz;
+    part[2 * nvblk + blockIdx.x] = dd;
   }
 }
 
@@ -506,6 +513,8 @@ struct SolveSys {
   size_t bytes = 0;
   double *b = nullptr, *xh = nullptr, *xt = nullptr, *rh = nullptr, *y = nullptr, *scal = nullptr, *part = nullptr;
   int bad = 0;              // consecutive updates whose FP32 correction was not finite
+  int64_t rollbacks = 0;    // updates that kept the previous solution (non-finite correction)
+  int64_t refreshes = 0;    // AMG updates whose z had to be recomputed (drift or r.z <= 0)
   void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;   // Krylov vectors (VT)
   float *binv = nullptr, *val32 = nullptr;
   int64_t npart = 0;
@@ -534,7 +543,7 @@ void BatchSolver::release(SolveSys* s) {
 size_t BatchSolver::bytes_needed(int64_t N, int64_t nadj, bool f64_vectors) {
   const int64_t n = 3 * N;
   const int64_t nsblk = spmv_blocks(N), nvblk = vec_blocks(N);
-  const int64_t npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
+  const int64_t npart = std::max<int64_t>(nsblk + 3 * nvblk, blocks(n)) + 64;
   return sizeof(double) * (5 * (size_t)n + SC_N + (size_t)npart) +
          (f64_vectors ? sizeof(double) : sizeof(float)) * 5 * (size_t)n +
          sizeof(float) * (9 * (size_t)N + 9 * (size_t)nadj) + 16 * 256;   // 14 buffers, 256-B aligned
@@ -565,12 +574,18 @@ int BatchSolver::update(SolveSys* s, bool init) {
     if (f64v_) k_ru_trial<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh, s->xt);
     else k_ru_trial<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh, s->xt);
     LLFEM_TRY(residual(s->xt, &r2));
+    // LLFEM_ENGINE_FORCE_ROLLBACK=k treats the k-th update's trial as not
+    // finite (a host-side switch for testing the rollback path)
+    const char* force_env = std::getenv("LLFEM_ENGINE_FORCE_ROLLBACK");
+    const int64_t force_rb = force_env ? (int64_t)std::atoll(force_env) : 0;
+    if (force_rb > 0 && s->updates + 1 == force_rb && s->rollbacks == 0) r2 = NAN;
     if (std::isfinite(r2)) {
       std::swap(s->xh, s->xt);
       s->bad = 0;
     } else {
       // the FP32 correction overflowed: keep xh, restart the recursion from it
       ++s->bad;
+      ++s->rollbacks;
       fresh = true;
       LLFEM_TRY(residual(s->xh, &r2));
     }
@@ -623,6 +638,26 @@ int BatchSolver::update(SolveSys* s, bool init) {
                                                         (float*)s->z, (float*)s->p, vpart);
     }
     k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
+    if (!fresh) {
+      // The rescaled z is consistent only if r.z > 0 and the true residual is
+      // close to the rescaled recursive one (relative drift below 1/4);
+      // otherwise recompute z = M r and restart the direction from it.
+      k_sum<<<1, kSB, 0, st>>>(nvblk, vpart + 2 * nvblk, s->scal + SC_N - 1);
+      double h[2] = {0.0, 0.0};   // r.z, drift^2
+      LLFEM_HIP(hipMemcpyAsync(&h[0], s->scal + SC_RHO, sizeof(double), hipMemcpyDeviceToHost, st));
+      LLFEM_HIP(hipMemcpyAsync(&h[1], s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+      LLFEM_HIP(hipStreamSynchronize(st));
+      // r is the true residual scaled to unit norm, so rr = 1 up to rounding
+      if (!(h[0] > 0.0) || !(h[1] <= kDriftAmg * kDriftAmg)) {
+        ++s->refreshes;
+        k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, 1, (float*)s->r,
+                                                          (float*)s->p, vpart);
+        LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
+        LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg,
+                                         (double)kMaxRunAmg);
+      }
+    }
     LLFEM_HIP(hipGetLastError());
     return LLFEM_OK;
   }
@@ -652,7 +687,7 @@ int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
   }
   const int64_t n = s->n, N = spec.N;
   const int64_t nsblk = spmv_blocks(N), nvblk = vec_blocks(N);
-  s->npart = std::max<int64_t>(nsblk + 2 * nvblk, blocks(n)) + 64;
+  s->npart = std::max<int64_t>(nsblk + 3 * nvblk, blocks(n)) + 64;   // SpMV, V-cycle r.z and drift partials
   char* m = static_cast<char*>(s->mem);
   auto take = [&](size_t bytes) -> char* {
     char* p = m;
@@ -857,6 +892,8 @@ bool BatchSolver::any_running() const {
 
 int64_t sys_iterations(const SolveSys* s) { return s->iters; }
 int64_t sys_updates(const SolveSys* s) { return s->updates; }
+int64_t sys_rollbacks(const SolveSys* s) { return s->rollbacks; }
+int64_t sys_refreshes(const SolveSys* s) { return s->refreshes; }
 int sys_status(const SolveSys* s) { return s->status; }
 double sys_relres(const SolveSys* s) { return s->relres; }
 

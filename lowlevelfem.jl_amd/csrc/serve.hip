@@ -125,7 +125,7 @@ struct Req {
   uint64_t support_key = 0;
   bool waiting = false;      // admitted, its AMG hierarchy still building
   int64_t cm_nodes = 0;      // mesh nodes (trace)
-  int64_t iterations = 0, updates = 0;
+  int64_t iterations = 0, updates = 0, rollbacks = 0, restarts = 0;
   double relres = 0.0;
 };
 
@@ -383,6 +383,8 @@ void complete(llfem_server* S, Req* q) {
   }
   q->iterations = sys_iterations(q->sys);
   q->updates = sys_updates(q->sys);
+  q->rollbacks = sys_rollbacks(q->sys);
+  q->restarts = sys_refreshes(q->sys);
   q->relres = sys_relres(q->sys);
   if (rc == LLFEM_OK && status == -2)
     rc = fail(LLFEM_ENOCONV, "cg: the FP32 correction was not finite in three successive updates");
@@ -620,6 +622,8 @@ extern "C" int llfem_server_wait(llfem_server* S, int64_t id, llfem_request_stat
     stats->service_ms = admitted ? ms(q->t_admit, q->t_done) : 0.0;
     stats->iterations = q->iterations;
     stats->updates = q->updates;
+    stats->rollbacks = q->rollbacks;
+    stats->restarts = q->restarts;
     stats->relres = q->relres;
     stats->mesh_cache_hit = q->cache_hit ? 1 : 0;
     stats->status = q->rc;

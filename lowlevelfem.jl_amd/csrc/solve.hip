@@ -485,6 +485,8 @@ int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* va
   const double relres = ff2 > 0 ? std::sqrt(rr2 / ff2) : std::sqrt(rr2);
   if (stats) {
     stats->iterations = total_it;
+    stats->rollbacks = 0;
+    stats->restarts = 0;
     stats->refinements = refinements;
     stats->relres = relres;
   }
@@ -649,6 +651,8 @@ int solve_host_reliable(llfem_ctx* ctx, SysSpec sp, bool f64_vectors, const doub
   const int status = sys_status(s);
   if (stats) {
     stats->iterations = sys_iterations(s);
+    stats->rollbacks = sys_rollbacks(s);
+    stats->restarts = sys_refreshes(s);
     stats->refinements = sys_updates(s);
     stats->relres = sys_relres(s);
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -724,6 +728,14 @@ int llfem_solve_csc(llfem_ctx* ctx, int64_t n, const int64_t* colptr, const int6
   const int64_t base = one_based ? 1 : 0;
   const int64_t nnz = colptr[n] - base;
   if (nnz < 0 || colptr[0] != base) return fail(LLFEM_EINVAL, "llfem_solve_csc: bad colptr");
+  // the device SpMV trusts these indices: check them here (O(n + nnz))
+  for (int64_t j = 0; j < n; ++j)
+    if (colptr[j + 1] < colptr[j])
+      return fail(LLFEM_EINVAL, "llfem_solve_csc: colptr is not non-decreasing at column " +
+                                    std::to_string(j + base));
+  for (int64_t k = 0; k < nnz; ++k)
+    if (rowval[k] < base || rowval[k] >= n + base)
+      return fail(LLFEM_EINVAL, "llfem_solve_csc: row index " + std::to_string(rowval[k]) + " out of range");
   LLFEM_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const size_t bytes = sizeof(int64_t) * (size_t)(n + 1) + sizeof(int64_t) * (size_t)nnz +

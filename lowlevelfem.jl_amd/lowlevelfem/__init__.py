@@ -303,25 +303,22 @@ def stiffnessMatrix(problem: Problem, elements=(), forceOneThread=True, gauss=No
     """stiffnessMatrix(problem) (src/linear.jl:70-87): Hooke material of each
     material group, rule "Gauss"*(2*order+1), CSC with dropzeros! applied
     (src/linear.jl:323-324).  Several groups: one device assembly per group
-    (element_scale 1 on its elements, 0 elsewhere), summed in group order."""
+    (element_scale 1 on its elements, 0 elsewhere), each added on the device
+    to the groups before it (accumulate), so the device copy is the full K."""
     if problem.type == "dummy":
         return None
     dev = problem.device()
     tab = _rule(problem, gauss)
     groups = _material_groups(problem)
-    val = None
-    for m, sc in groups:
-        v, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision],
-                             element_scale=sc)
-        val = v if val is None else val + v
+    last = len(groups) - 1
+    for i, (m, sc) in enumerate(groups):
+        val, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision],
+                               element_scale=sc, accumulate=i > 0, copy_out=i == last)
     colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
     A = _csc(problem.ndofs(), colptr, rowval, val)
     if dropzeros_:
         A = dropzeros(A)
-    if len(groups) == 1:
-        return SystemMatrix(A, problem, _device_k(problem, st))
-    problem._k_version = getattr(problem, "_k_version", 0) + 1   # the device copy holds one group only
-    return SystemMatrix(A, problem, st.as_dict())
+    return SystemMatrix(A, problem, _device_k(problem, st))
 
 
 def _device_k(problem: Problem, st) -> dict:

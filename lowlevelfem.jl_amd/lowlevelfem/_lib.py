@@ -32,7 +32,8 @@ class OpDesc(C.Structure):
     _fields_ = [("op", C.c_int32), ("q", C.c_int32), ("weights", C.c_void_p),
                 ("N_ref", C.c_void_p), ("dN_ref", C.c_void_p), ("D", C.c_void_p),
                 ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
-                ("lumped", C.c_int32), ("precision", C.c_int32), ("element_scale", C.c_void_p)]
+                ("lumped", C.c_int32), ("precision", C.c_int32), ("element_scale", C.c_void_p),
+                ("accumulate", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -40,7 +41,8 @@ class Stats(C.Structure):
                 ("total_ms", C.c_double), ("nnz", C.c_int64), ("elements", C.c_int64),
                 ("alg_bytes", C.c_double), ("iterations", C.c_int64),
                 ("refinements", C.c_int64), ("relres", C.c_double),
-                ("alg_flops", C.c_double), ("setup_ms", C.c_double)]
+                ("alg_flops", C.c_double), ("setup_ms", C.c_double),
+                ("rollbacks", C.c_int64), ("restarts", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -64,7 +66,8 @@ class RequestSpec(C.Structure):   # llfem_request
 class RequestStats(C.Structure):  # llfem_request_stats
     _fields_ = [("latency_ms", C.c_double), ("queue_ms", C.c_double), ("service_ms", C.c_double),
                 ("iterations", C.c_int64), ("updates", C.c_int64), ("relres", C.c_double),
-                ("mesh_cache_hit", C.c_int32), ("status", C.c_int32)]
+                ("mesh_cache_hit", C.c_int32), ("status", C.c_int32),
+                ("rollbacks", C.c_int64), ("restarts", C.c_int64)]
 
 
 class LLFEMError(RuntimeError):
@@ -224,12 +227,15 @@ class DeviceMesh:
         return colptr, rowval
 
     def assemble(self, op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False,
-                 precision=FP64, copy_out=True, element_scale=None, out=None):
+                 precision=FP64, copy_out=True, element_scale=None, out=None, accumulate=False):
         """Values to a new host array (copy_out), to `out` (a float64 array of
-        the right size), or left on the device (copy_out=False)."""
+        the right size), or left on the device (copy_out=False).  With
+        accumulate=True the values are added to the mesh's resident operator
+        and the sum is what is returned."""
         if element_scale is not None and len(element_scale) != self.E:
             raise ValueError(f"element_scale has {len(element_scale)} entries, the mesh {self.E} elements")
         desc, _keep = make_desc(op, tables, E, nu, rho, D, lumped, precision, element_scale)
+        desc.accumulate = int(bool(accumulate))
         st = Stats()
         n = 3 * self.N if (op == OP_MASS and lumped) else self.pattern(op, with_arrays=False)
         if out is not None:

@@ -112,3 +112,30 @@ def test_amg_inhomogeneous_dirichlet(gpu_ctx):
     fk = K[:, fixed] @ u[fixed]
     assert np.linalg.norm((K @ u - f)[free]) / np.linalg.norm(fk[free]) < 1e-10
     dev.close()
+
+
+def test_rollback_and_refresh_are_counted(gpu_ctx):
+    """The reliable-update engine keeps the previous FP64 solution when an FP32
+    correction is not finite (a rollback) and recomputes the AMG
+    preconditioned residual when the rescaled one is no longer consistent (a
+    restart).  A rollback is forced from the host at the second update: the
+    solve still converges to the same displacements, and the counters say
+    what happened; an unforced solve reports no rollback."""
+    import os
+    say("[amg] forced rollback")
+    m, tab, f, fixed = _system(dict(lx=1.0, ly=1.0, lz=3.0, cells=(4, 4, 12), elem="tet", order=2), 2)
+    dev = _lib.DeviceMesh(gpu_ctx, m.coords, m.etype, m.conn)
+    dev.assemble(_lib.OP_STIFFNESS, tab, copy_out=False)
+    u0, st0 = dev.solve(f, fixed, method=_lib.PCG_AMG_RELIABLE, reltol=1e-10)
+    assert st0.rollbacks == 0 and st0.relres <= 1e-10
+    os.environ["LLFEM_ENGINE_FORCE_ROLLBACK"] = "2"
+    try:
+        u1, st1 = dev.solve(f, fixed, method=_lib.PCG_AMG_RELIABLE, reltol=1e-10)
+    finally:
+        del os.environ["LLFEM_ENGINE_FORCE_ROLLBACK"]
+    say(f"  rollbacks {st1.rollbacks}, restarts {st1.restarts} (unforced {st0.restarts}), "
+        f"updates {st1.refinements}, relres {st1.relres:.2e}")
+    assert st1.rollbacks == 1
+    assert st1.relres <= 1e-10
+    assert np.linalg.norm(u1 - u0) <= 1e-8 * np.linalg.norm(u0)
+    assert st0.restarts >= 0 and st1.restarts >= 0

@@ -189,6 +189,30 @@ def test_several_material_groups():
     assert np.array_equal(K.A.indptr, cp) and np.array_equal(K.A.indices, rv)
     assert np.abs(K.A.data - Kref).max() <= 1e-12 * np.abs(Kref).max()
     assert spla.norm(M.A - Mref) / spla.norm(Mref) < 1e-12
+    # the groups were added on the device: the mesh holds the whole K, and the
+    # AMG solve (which solves that device copy) matches a direct solve of it
+    dev = P.device()
+    assert np.array_equal(dev.values(_lib.OP_STIFFNESS), K.A.data)
+    sup = [lf.displacementConstraint("left", ux=0, uy=0, uz=0)]
+    f = lf.loadVector(P, [lf.load("right", fy=-1.0)])
+    u = lf.solveDisplacement(K, f, support=sup, iterative=True, reltol=1e-10, preconditioner="amg")
+    fixed = lf.constrainedDoFs(P, sup)
+    free = np.setdiff1d(np.arange(P.ndofs()), fixed)
+    Kff = K.A[free][:, free].tocsc()
+    uref = spla.spsolve(Kff, f.a[free, 0])
+    assert np.linalg.norm(u.a[free, 0] - uref) <= 1e-8 * np.linalg.norm(uref)
+    # one group alone leaves an incomplete operator that llfem_solve refuses;
+    # accumulating the other group completes it
+    sc = np.zeros(m.E)
+    sc[m.volumes["steel"]] = 1.0
+    dev.assemble(_lib.OP_STIFFNESS, tab, E=steel.E, nu=steel.nu, element_scale=sc, copy_out=False)
+    with pytest.raises(_lib.LLFEMError, match="no stiffness assembled"):
+        dev.values(_lib.OP_STIFFNESS)
+    sc2 = np.zeros(m.E)
+    sc2[m.volumes["alu"]] = 1.0
+    Kd, _ = dev.assemble(_lib.OP_STIFFNESS, tab, E=alu.E, nu=alu.nu, element_scale=sc2, accumulate=True)
+    assert np.abs(Kd - Kref).max() <= 1e-12 * np.abs(Kref).max()
+    assert np.array_equal(dev.values(_lib.OP_STIFFNESS), Kd)
     # one material named for a group that holds every element: the plain path
     m.volumes["all"] = np.arange(m.E)
     P1 = lf.Problem([lf.Material("all", E=2.1e5, nu=0.3)])
@@ -202,3 +226,33 @@ def test_several_material_groups():
     assert np.array_equal(K1.A.data, K2.A.data)
     with pytest.raises(_lib.LLFEMError, match="element_scale must be finite and >= 0"):
         K1.model.device().assemble(_lib.OP_STIFFNESS, tab, element_scale=-np.ones(m.E))
+
+
+def test_solve_csc_rejects_bad_indices():
+    """llfem_solve_csc checks colptr and rowval before they reach the device
+    SpMV: a row index out of range or a decreasing colptr is LLFEM_EINVAL."""
+    import ctypes as C
+    say("[api] solve_csc index checks")
+    ctx = _lib.default_context()
+    lib = ctx.lib
+    n = 3
+    A = sp.csc_matrix(np.array([[4.0, 1, 0], [1, 4, 1], [0, 1, 4]]))
+    f = np.ones(n)
+    u = np.zeros(n)
+
+    def call(cp, rv):
+        cp = np.ascontiguousarray(cp, dtype=np.int64)
+        rv = np.ascontiguousarray(rv, dtype=np.int64)
+        return lib.llfem_solve_csc(ctx.h, n, _lib.ptr(cp), _lib.ptr(rv), _lib.ptr(A.data), 0, _lib.ptr(f),
+                                   None, 0, None, _lib.PCG_JACOBI_FP64, C.c_double(1e-10), 100, _lib.ptr(u),
+                                   None)
+    assert call(A.indptr, A.indices) == _lib.OK
+    assert np.allclose(A @ u, f)
+    bad = A.indices.copy()
+    bad[2] = 7
+    assert call(A.indptr, bad) == _lib.EINVAL
+    assert "out of range" in lib.llfem_last_error().decode()
+    badp = A.indptr.copy()
+    badp[1], badp[2] = badp[2], badp[1]
+    assert call(badp, A.indices) == _lib.EINVAL
+    assert "non-decreasing" in lib.llfem_last_error().decode()
