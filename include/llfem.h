@@ -218,6 +218,25 @@ LLFEM_API int llfem_solve_csc(llfem_ctx* ctx, int64_t ndof, const int64_t* colpt
                     const int64_t* fixed, int64_t nfixed, const double* u_fixed, int32_t method,
                     double reltol, int64_t maxiter, double* u_out, llfem_stats* stats);
 
+/* Allocation-free variant of llfem_solve_csc with LLFEM_PCG_JACOBI_FP64.  It
+ * has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory during the call (tight serving loops, code that
+ * forbids hipMalloc).  Every argument is DEVICE memory the caller owns:
+ * zero-based colptr [ndof+1] (int64), rowval [nnz] (int32), nzval [nnz],
+ * f [ndof], fixed [ndof] (1 = constrained DoF), u_fixed [ndof] (values on the
+ * constrained DoFs; NULL = zeros), u_out [ndof].  The solver's work vectors
+ * live in one static device array per GPU (64 MiB, held until the process
+ * ends), so ndof is limited to about 1,000,000; larger systems return
+ * LLFEM_EINVAL with the limit in llfem_last_error.  Index errors give the
+ * same LLFEM_EINVAL as llfem_solve_csc.  Calls on one GPU share that array:
+ * within a process they are serialised by a lock; calls from different
+ * processes on one GPU each have their own copy.  Results are bit-identical
+ * to llfem_solve_csc(..., LLFEM_PCG_JACOBI_FP64, ...) on the same system. */
+LLFEM_API int llfem_solve_csc_noalloc(llfem_ctx* ctx, int64_t ndof, const int64_t* colptr,
+                                      const int32_t* rowval, const double* nzval, const double* f,
+                                      const uint8_t* fixed, const double* u_fixed, double reltol,
+                                      int64_t maxiter, double* u_out, llfem_stats* stats);
+
 /* ---- batched assembly ----------------------------------------------------
  * Replaces a host loop of stiffnessMatrix(problem) / massMatrix(problem;
  * lumped) (src/linear.jl:70-87, 1843-2004) over independent problems: all

@@ -310,4 +310,10 @@ int64_t sys_refreshes(const SolveSys* s);   // AMG updates that recomputed z
 int sys_status(const SolveSys* s);      // 0 running, 1 converged, -1 maxiter, -2 non-finite FP32 correction
 double sys_relres(const SolveSys* s);
 
+// Size of the per-device workspace array of llfem_solve_csc_noalloc, and its
+// set-up: looks up the array's device address on `device` (current), once
+// per device; called by llfem_ctx_create.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+int noalloc_scratch_setup(int device);
+
 }  // namespace llfem

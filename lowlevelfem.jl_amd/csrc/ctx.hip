@@ -129,6 +129,8 @@ int llfem_ctx_create(int device, size_t hbm_budget_bytes, llfem_ctx** out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(LLFEM_EUNSUP, std::string("libllfem is built for gfx950; device is ") +
                                   prop.gcnArchName);
+  // device address of the allocation-free solve's workspace, once per device
+  LLFEM_TRY(llfem::noalloc_scratch_setup(device));
   llfem_ctx* c = new llfem_ctx();
   c->device = device;
   c->budget = hbm_budget_bytes;

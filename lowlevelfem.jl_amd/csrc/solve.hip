@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 
@@ -381,26 +382,35 @@ int pcg(llfem_ctx* ctx, const Op& K, const MT* val, const uint8_t* fixed, const 
   return LLFEM_OK;
 }
 
+__global__ void k_zero(int64_t n, double* __restrict__ a) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
+// Device workspace of solve_device_ws: 8 FP64 vectors of K.ndof(), the
+// reduction partials and scalars, and 7 FP32 vectors for the mixed method.
+template <typename Op>
+size_t solve_ws_bytes(llfem_ctx* ctx, const Op& K, int method) {
+  const int64_t n = K.ndof();
+  const int64_t nparts = std::max<int64_t>(K.grid(), 2 * (int64_t)vgrid(n, ctx)) + 2;
+  const size_t nf = method == LLFEM_PCG_MIXED_REFINE ? 7 : 0;
+  return sizeof(double) * (8 * (size_t)n + (size_t)nparts + 8 + S_NS + 8) + sizeof(float) * (nf * (size_t)n + 8);
+}
+
 // Full solve on device data: f, ufix (nullable) and u are device vectors of
 // K.ndof(); fixed is a device 0/1 mask.  val32 is needed for the mixed method.
+// `base` holds solve_ws_bytes(ctx, K, method) bytes; every part of it is
+// written before it is read, so it may hold anything on entry.
 template <typename Op>
-int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* val32,
-                 const uint8_t* fixed, const double* f, const double* ufix, int method, double reltol,
-                 int64_t maxiter, double* u, llfem_stats* stats) {
+int solve_device_ws(llfem_ctx* ctx, const Op& K, const double* val, const float* val32,
+                    const uint8_t* fixed, const double* f, const double* ufix, int method, double reltol,
+                    int64_t maxiter, double* u, llfem_stats* stats, void* base) {
   hipStream_t st = ctx->stream;
   const int64_t n = K.ndof();
   const unsigned gv = vgrid(n, ctx);
   const int64_t nparts = std::max<int64_t>(K.grid(), 2 * (int64_t)gv) + 2;
   const bool mixed = (method == LLFEM_PCG_MIXED_REFINE);
-  const size_t nd = 8, nf = mixed ? 7 : 0;
-  size_t bytes = sizeof(double) * (nd * (size_t)n + (size_t)nparts + 8 + S_NS + 8) +
-                 sizeof(float) * (nf * (size_t)n + 8);
-  void* base = nullptr;
-  LLFEM_TRY(dmalloc(ctx, &base, bytes));
-  struct Free {
-    llfem_ctx* c; void* p; size_t b;
-    ~Free() { (void)hipStreamSynchronize(c->stream); dfree(c, p, b); }
-  } guard{ctx, base, bytes};
   double* b = (double*)base;
   double* x = b + n;
   double* r = x + n;
@@ -437,7 +447,7 @@ int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* va
     k_axpy<<<gv, kBS, 0, st>>>(n, 1.0, x, u);
   } else {
     // x accumulates the free-DoF correction in FP64; r is the FP64 residual
-    LLFEM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+    k_zero<<<gv, kBS, 0, st>>>(n, x);
     LLFEM_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double rnorm = bnorm;
     const double inner_floor = 1e-5;
@@ -475,7 +485,7 @@ int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* va
   LLFEM_HIP(hipMemcpyAsync(&rr2, S.red, sizeof(double), hipMemcpyDeviceToHost, st));
   LLFEM_HIP(hipStreamSynchronize(st));
   // ||f_free||
-  LLFEM_HIP(hipMemsetAsync(y, 0, sizeof(double) * n, st));
+  k_zero<<<gv, kBS, 0, st>>>(n, y);
   k_resid<<<gv, kBS, 0, st>>>(n, f, y, fixed, z, S.part);
   k_reduce<<<1, kBS, 0, st>>>(gv, 1, S.part, S.red);
   double ff2 = 0.0;
@@ -493,6 +503,21 @@ int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* va
   if (!converged)
     return fail(LLFEM_ENOCONV, "cg: maxiter reached (relative residual " + std::to_string(relres) + ")");
   return LLFEM_OK;
+}
+
+// solve_device_ws on a workspace allocated for this call.
+template <typename Op>
+int solve_device(llfem_ctx* ctx, const Op& K, const double* val, const float* val32,
+                 const uint8_t* fixed, const double* f, const double* ufix, int method, double reltol,
+                 int64_t maxiter, double* u, llfem_stats* stats) {
+  const size_t bytes = solve_ws_bytes(ctx, K, method);
+  void* base = nullptr;
+  LLFEM_TRY(dmalloc(ctx, &base, bytes));
+  struct Free {
+    llfem_ctx* c; void* p; size_t b;
+    ~Free() { (void)hipStreamSynchronize(c->stream); dfree(c, p, b); }
+  } guard{ctx, base, bytes};
+  return solve_device_ws(ctx, K, val, val32, fixed, f, ufix, method, reltol, maxiter, u, stats, base);
 }
 
 __global__ void k_set_fixed(int64_t nfix, const int64_t* __restrict__ idx, int64_t base, int64_t n,
@@ -678,6 +703,48 @@ int ensure_k32(llfem_mesh* m) {
   return LLFEM_OK;
 }
 
+// ---- workspace of the allocation-free solve ------------------------------
+// One array per GPU, part of this library's device image: the runtime places
+// it when the image is loaded on a device and it stays until the process
+// ends.  In host code `scratch` names a host-side stand-in, never the device
+// memory: host code uses only the address hipGetSymbolAddress returns for
+// the current device, kept per device in g_scratch.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+  std::mutex call;          // one allocation-free solve at a time on the device
+  void* ptr = nullptr;      // device address of `scratch` on this device
+};
+ScratchSlot g_scratch[kMaxDevices];
+std::mutex g_scratch_setup;
+
+__global__ void k_check_csc(int64_t n, const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                            int64_t nnz, int* __restrict__ err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+    if (colptr[j + 1] < colptr[j]) atomicOr(err, 1);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += stride)
+    if (rowval[k] < 0 || (int64_t)rowval[k] >= n) atomicOr(err, 2);
+}
+
+__global__ void k_zero_int(int* a) { *a = 0; }
+}  // namespace
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(LLFEM_EUNSUP, "device ordinal " + std::to_string(device) + " above the supported maximum");
+  std::lock_guard<std::mutex> lk(g_scratch_setup);
+  if (g_scratch[device].ptr) return LLFEM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(LLFEM_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch[device].ptr = p;
+  return LLFEM_OK;
+}
+
 }  // namespace llfem
 
 using namespace llfem;
@@ -761,6 +828,53 @@ int llfem_solve_csc(llfem_ctx* ctx, int64_t n, const int64_t* colptr, const int6
   CscOp K{n, d_cp, d_rv};
   return solve_host(ctx, K, d_val, d_val32, n, f, fixed, nfixed, one_based, u_fixed, method, reltol,
                     maxiter, u_out, stats);
+}
+
+int llfem_solve_csc_noalloc(llfem_ctx* ctx, int64_t n, const int64_t* d_colptr, const int32_t* d_rowval,
+                            const double* d_nzval, const double* d_f, const uint8_t* d_fixed,
+                            const double* d_u_fixed, double reltol, int64_t maxiter, double* d_u_out,
+                            llfem_stats* stats) {
+  if (!ctx || !d_colptr || !d_rowval || !d_nzval || !d_f || !d_fixed || !d_u_out)
+    return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: NULL argument");
+  if (n <= 0 || n >= ((int64_t)1 << 31)) return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: bad size");
+  if (!(reltol > 0)) return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: reltol must be positive");
+  CscOp K{n, d_colptr, d_rowval};
+  // the flag word of the index check sits after the solver's workspace
+  const size_t ws = (solve_ws_bytes(ctx, K, LLFEM_PCG_JACOBI_FP64) + 255) & ~(size_t)255;
+  if (ws + 256 > kNoAllocScratchBytes)
+    return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: ndof " + std::to_string(n) + " needs " +
+                                  std::to_string(ws + 256) + " bytes of workspace; the maximum is " +
+                                  std::to_string(kNoAllocScratchBytes) + " (about " +
+                                  std::to_string(kNoAllocScratchBytes / 65) + " DoFs)");
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev].ptr)
+    return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: no workspace on device " + std::to_string(dev));
+  LLFEM_HIP(hipSetDevice(dev));
+  std::lock_guard<std::mutex> lk(g_scratch[dev].call);
+  unsigned char* base = (unsigned char*)g_scratch[dev].ptr;
+  int* d_err = (int*)(base + ws);
+  hipStream_t st = ctx->stream;
+  auto t0 = std::chrono::steady_clock::now();
+  // the same index checks as llfem_solve_csc, on the device
+  int64_t ends[2] = {0, 0};
+  LLFEM_HIP(hipMemcpyAsync(&ends[0], d_colptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipMemcpyAsync(&ends[1], d_colptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  if (ends[0] != 0 || ends[1] < 0) return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: bad colptr");
+  k_zero_int<<<1, 1, 0, st>>>(d_err);
+  k_check_csc<<<vgrid(std::max<int64_t>(n, ends[1]), ctx), kBS, 0, st>>>(n, d_colptr, d_rowval, ends[1], d_err);
+  int herr = 0;
+  LLFEM_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  LLFEM_HIP(hipStreamSynchronize(st));
+  if (herr & 1) return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: colptr is not non-decreasing");
+  if (herr & 2) return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: row index out of range");
+  const int s = solve_device_ws(ctx, K, d_nzval, nullptr, d_fixed, d_f, d_u_fixed, LLFEM_PCG_JACOBI_FP64,
+                                reltol, maxiter, d_u_out, stats, base);
+  if (s != LLFEM_OK && s != LLFEM_ENOCONV) return s;
+  LLFEM_HIP(hipStreamSynchronize(st));
+  if (stats)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return s;
 }
 
 }  // extern "C"

@@ -22,7 +22,7 @@ EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
     "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
     "llfem_assemble", "llfem_load_traction", "llfem_face_points", "llfem_load_traction_gp",
-    "llfem_solve", "llfem_solve_csc",
+    "llfem_solve", "llfem_solve_csc", "llfem_solve_csc_noalloc",
     "llfem_assemble_batch", "llfem_mesh_values", "llfem_autotune",
     "llfem_server_create", "llfem_server_submit", "llfem_server_wait", "llfem_server_destroy",
 ]
@@ -104,6 +104,7 @@ def load_library(path: str = LIB_PATH):
     lib.llfem_solve.argtypes = [P, P, P, I64, I32, P, I32, D, I64, P, C.POINTER(Stats)]
     lib.llfem_solve_csc.argtypes = [P, I64, P, P, P, I32, P, P, I64, P, I32, D, I64, P,
                                     C.POINTER(Stats)]
+    lib.llfem_solve_csc_noalloc.argtypes = [P, I64, P, P, P, P, P, P, D, I64, P, C.POINTER(Stats)]
     lib.llfem_assemble_batch.argtypes = [I32, P, P, C.POINTER(Stats)]
     lib.llfem_mesh_values.argtypes = [P, I32, I32, P]
     lib.llfem_autotune.argtypes = [P, C.POINTER(OpDesc), C.POINTER(TuneResult)]
@@ -332,6 +333,16 @@ def solve_csc(ctx: Context, colptr, rowval, nzval, f, fixed, ufix=None, method=P
                                   fx.size, ptr(uf), method, float(reltol), int(maxiter), ptr(u),
                                   C.byref(st)))
     return u, st
+
+
+def solve_csc_noalloc(ctx: Context, ndof, d_colptr, d_rowval, d_nzval, d_f, d_fixed, d_ufix, d_u,
+                      reltol=1e-10, maxiter=100000):
+    """llfem_solve_csc_noalloc on device addresses (ints): zero-based int64
+    colptr, int32 rowval, FP64 nzval/f/u_fixed/u, uint8 fixed mask."""
+    st = Stats()
+    check(ctx.lib.llfem_solve_csc_noalloc(ctx.h, int(ndof), d_colptr, d_rowval, d_nzval, d_f, d_fixed,
+                                          d_ufix, float(reltol), int(maxiter), d_u, C.byref(st)))
+    return st
 
 
 class Server:
