@@ -196,6 +196,41 @@ LLFEM_API int llfem_load_traction_gp(llfem_mesh* mesh, int32_t face_type, int32_
                                      const double* N_ref, const double* dN_ref, const double* t_gp,
                                      double* f_out);
 
+/* Body force on volume elements of the mesh: replaces loadVector(problem,
+ * [load(volume group, fx, fy, fz)]) ("Volume -> body force",
+ * src/linear.jl:3054-3068, integrated by :3240-3413 with Ja = det J) and the
+ * DSL linear form  ∫(Pu ⋅ g; Ω)  (assemble_linear, src/multifield.jl:3496-3748).
+ * f_a += sum_q N_a(xi_q) g(x_q) det J_q w_q over the elements elems[0..ne)
+ * (0-based indices into the mesh's elements; elems = NULL with ne = 0: all
+ * elements), summed per node in ascending element order.  g is the constant
+ * force [3] when g_gp is NULL, else g_gp [ne][q][3] gives it per Gauss point
+ * (a function of (x, y, z) evaluated by the caller at llfem_volume_points).
+ * f_out is [3N], zero-initialised here. */
+LLFEM_API int llfem_load_body(llfem_mesh* mesh, int32_t q, const double* weights, const double* N_ref,
+                              const double* dN_ref, const int64_t* elems, int64_t ne, const double g[3],
+                              const double* g_gp, double* f_out);
+
+/* Physical coordinates of the volume Gauss points of elems[0..ne) (NULL, 0:
+ * all elements): xyz_out is [ne][q][3], x_q = sum_a N_a(xi_q) x_a. */
+LLFEM_API int llfem_volume_points(llfem_mesh* mesh, int32_t q, const double* N_ref, const int64_t* elems,
+                                  int64_t ne, double* xyz_out);
+
+/* Pressure p on a face set of a 3D solid: replaces loadVector(problem,
+ * [load(name, p = p)]) (src/linear.jl:3196-3224).  As the reference does it:
+ * the unit normal n = t1 x t2 / |t1 x t2| of each face at each of its nodes
+ * (normalVector, src/general.jl:4848-4940; dN_nodes [nf][nf][3] are the face
+ * basis gradients at the face's own reference nodes), the nodal traction
+ * t_b = (sum over the faces at node b of (-n) p_b) / (number of such faces)
+ * (elementsToNodes, src/general.jl:5519-5640), interpolated at the Gauss
+ * points, t(x_q) = sum_b N_b t_b (_loadvec_helper, src/linear.jl:3033-3044),
+ * and f_a += sum_q N_a t(x_q) |t1 x t2| w_q.  p_b = p_nodal[b] ([N], the
+ * caller's function at the node coordinates) or the constant p when p_nodal
+ * is NULL.  f_out is [3N], zero-initialised here. */
+LLFEM_API int llfem_load_pressure(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
+                                  int64_t F, int32_t one_based, int32_t q, const double* weights,
+                                  const double* N_ref, const double* dN_ref, const double* dN_nodes, double p,
+                                  const double* p_nodal, double* f_out);
+
 /* ---- solve -------------------------------------------------------------
  * Replaces solveDisplacement(K, f; support, iterative=true, reltol, maxiter,
  * preconditioner) (src/linear.jl:3711-3743) on the stiffness most recently

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -312,8 +313,11 @@ double sys_relres(const SolveSys* s);
 
 // Size of the per-device workspace array of llfem_solve_csc_noalloc, and its
 // set-up: looks up the array's device address on `device` (current), once
-// per device; called by llfem_ctx_create.
+// per device; called by llfem_ctx_create.  noalloc_scratch gives that
+// address and the lock that serialises its users on the device (false
+// before set-up).
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 int noalloc_scratch_setup(int device);
+bool noalloc_scratch(int device, void** ptr, std::mutex** lock);
 
 }  // namespace llfem

@@ -1,4 +1,6 @@
 // Context, error reporting and device-memory accounting.
+#include <mutex>
+
 #include "common.h"
 
 #include <algorithm>
@@ -105,6 +107,52 @@ uint64_t hash_bytes(const void* p, size_t n, uint64_t h) {
   r *= kC;
   r ^= r >> 29;
   return r;
+}
+
+}  // namespace llfem
+
+// ---- workspace of the allocation-free solve ------------------------------
+// One array per GPU, part of this library's device image: the runtime places
+// it when the image is loaded on a device and it stays until the process
+// ends.  In host code `scratch` names a host-side stand-in, never the device
+// memory: host code uses only the address hipGetSymbolAddress returns for
+// the current device, kept per device in g_scratch.
+static __device__ __align__(256) unsigned char scratch[llfem::kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+  std::mutex call;          // one allocation-free solve at a time on the device
+  void* ptr = nullptr;      // device address of `scratch` on this device
+};
+ScratchSlot g_scratch[kMaxDevices];
+std::mutex g_scratch_setup;
+}  // namespace
+
+namespace llfem {
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(LLFEM_EUNSUP, "device ordinal " + std::to_string(device) + " above the supported maximum");
+  std::lock_guard<std::mutex> lk(g_scratch_setup);
+  if (g_scratch[device].ptr) return LLFEM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(LLFEM_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch[device].ptr = p;
+  return LLFEM_OK;
+}
+
+bool noalloc_scratch(int device, void** ptr, std::mutex** lock) {
+  if (device < 0 || device >= kMaxDevices) return false;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_setup);
+    if (!g_scratch[device].ptr) return false;
+    *ptr = g_scratch[device].ptr;
+  }
+  *lock = &g_scratch[device].call;
+  return true;
 }
 
 }  // namespace llfem

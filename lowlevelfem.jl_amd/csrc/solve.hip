@@ -703,23 +703,7 @@ int ensure_k32(llfem_mesh* m) {
   return LLFEM_OK;
 }
 
-// ---- workspace of the allocation-free solve ------------------------------
-// One array per GPU, part of this library's device image: the runtime places
-// it when the image is loaded on a device and it stays until the process
-// ends.  In host code `scratch` names a host-side stand-in, never the device
-// memory: host code uses only the address hipGetSymbolAddress returns for
-// the current device, kept per device in g_scratch.
-static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
-
 namespace {
-constexpr int kMaxDevices = 64;
-struct ScratchSlot {
-  std::mutex call;          // one allocation-free solve at a time on the device
-  void* ptr = nullptr;      // device address of `scratch` on this device
-};
-ScratchSlot g_scratch[kMaxDevices];
-std::mutex g_scratch_setup;
-
 __global__ void k_check_csc(int64_t n, const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
                             int64_t nnz, int* __restrict__ err) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -731,19 +715,6 @@ __global__ void k_check_csc(int64_t n, const int64_t* __restrict__ colptr, const
 
 __global__ void k_zero_int(int* a) { *a = 0; }
 }  // namespace
-
-int noalloc_scratch_setup(int device) {
-  if (device < 0 || device >= kMaxDevices)
-    return fail(LLFEM_EUNSUP, "device ordinal " + std::to_string(device) + " above the supported maximum");
-  std::lock_guard<std::mutex> lk(g_scratch_setup);
-  if (g_scratch[device].ptr) return LLFEM_OK;
-  void* p = nullptr;
-  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-  if (e != hipSuccess || !p)
-    return fail(LLFEM_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
-  g_scratch[device].ptr = p;
-  return LLFEM_OK;
-}
 
 }  // namespace llfem
 
@@ -847,11 +818,13 @@ int llfem_solve_csc_noalloc(llfem_ctx* ctx, int64_t n, const int64_t* d_colptr, 
                                   std::to_string(kNoAllocScratchBytes) + " (about " +
                                   std::to_string(kNoAllocScratchBytes / 65) + " DoFs)");
   const int dev = ctx->device;
-  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev].ptr)
+  void* ws_ptr = nullptr;
+  std::mutex* ws_lock = nullptr;
+  if (!noalloc_scratch(dev, &ws_ptr, &ws_lock))
     return fail(LLFEM_EINVAL, "llfem_solve_csc_noalloc: no workspace on device " + std::to_string(dev));
   LLFEM_HIP(hipSetDevice(dev));
-  std::lock_guard<std::mutex> lk(g_scratch[dev].call);
-  unsigned char* base = (unsigned char*)g_scratch[dev].ptr;
+  std::lock_guard<std::mutex> lk(*ws_lock);
+  unsigned char* base = (unsigned char*)ws_ptr;
   int* d_err = (int*)(base + ws);
   hipStream_t st = ctx->stream;
   auto t0 = std::chrono::steady_clock::now();

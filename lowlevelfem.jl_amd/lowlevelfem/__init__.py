@@ -11,7 +11,7 @@ hot path (LowLevelFEM.jl 1.13.31):
     VectorField          src/general.jl:1554-1563 (container only)
     stiffnessMatrix      src/linear.jl:70-87 -> stiffnessMatrixSolid :190-330
     massMatrix           src/linear.jl:1843-2004
-    loadVector           src/linear.jl:3088-3434 (traction loads)
+    loadVector           src/linear.jl:3088-3434 (traction, pressure, body force)
     solveDisplacement    src/linear.jl:3711-3743
     displacementConstraint / load / BoundaryCondition / LoadCondition
                          src/general.jl:2480-2605, 4179-4233
@@ -20,7 +20,8 @@ hot path (LowLevelFEM.jl 1.13.31):
     build_csc_pattern    src/multifield.jl:2083-2250
     SymGrad / Id / integral
                          the ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); ...) and ∫(Id(Pu) ⋅ c ⋅ Id(Pu); ...)
-                         DSL paths, src/multifield.jl:5295-5391, assemble_operator :2630-3280
+                         DSL paths, src/multifield.jl:5295-5391, assemble_operator :2630-3280;
+                         ∫(Pu ⋅ g; Ω / Γ), assemble_linear src/multifield.jl:3496-3748
 
 Differences that follow from the host language: DoF and node indices are
 0-based (dof = 3*node + c); sparse matrices are scipy.sparse.csc_matrix with
@@ -377,13 +378,20 @@ def _integral_id(a, c, b, gauss, csc_matrix):
     return SystemMatrix(K, P, st.as_dict())
 
 
-def integral(a, D, b, gauss=None, csc_matrix=None, precision="fp64", threads=None):
+def integral(a, D, b=None, gauss=None, csc_matrix=None, precision="fp64", threads=None, Γ=None, Ω=None):
     """∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu); gauss, csc_matrix) (src/multifield.jl:5295-5339).
     D is the 6x6 coefficient in SymGrad row order [xx,yy,zz,xy,xz,yz].
     Without csc_matrix the result gets dropzeros! (:5336); with it the
     structural pattern is kept and the caller's matrix is filled in place.
     integral(Id(Pu), c, Id(Pu), ...) is the mass-shaped form (scalar c).
+    integral(Pu, g; Γ=name or Ω=name) -- no third operand -- is the linear
+    form ∫(Pu ⋅ g) with g a 3-vector or a function of (x, y, z).
     ``threads`` (the reference's worker count) has no meaning on the GPU path."""
+    if b is None:
+        P = a.P if isinstance(a, Id) else a
+        if not isinstance(P, Problem):
+            raise NotImplementedError("only ∫(Pu ⋅ g) is on the GPU path as a linear form")
+        return _linear_form(P, D, gauss=gauss, Γ=Γ, Ω=Ω)
     if isinstance(a, Id) or isinstance(b, Id):
         return _integral_id(a, D, b, gauss, csc_matrix)
     if not isinstance(a, SymGrad) or not isinstance(b, SymGrad) or a.P is not b.P:
@@ -436,10 +444,64 @@ def _eval_at(fn, P):
         return np.array([float(fn(*x)) for x in flat]).reshape(shape)
 
 
+def _volume_elems(problem: Problem, name: str):
+    """Element indices of volume group `name` (None: every element, the
+    generated box's one group "body")."""
+    vols = problem.mesh.volumes
+    if name in vols:
+        return np.asarray(vols[name], dtype=np.int64)
+    if name == "body" and not vols:
+        return None
+    raise KeyError(f"Physical group '{name}' not found.")
+
+
+def _eval_vec_at(g, P):
+    """A vector coefficient at the Gauss points P (n, q, 3): a 3-vector of
+    numbers, or a function of (x, y, z) returning one (called on whole
+    arrays when it accepts them, else point by point)."""
+    shape = P.shape[:2]
+    if not callable(g):
+        v = np.asarray(g, dtype=np.float64).reshape(-1)
+        if v.size != 3:
+            raise ValueError(f"assemble_linear: vector coefficient length ({v.size}) does not match "
+                             "operator output dimension (3).")
+        return np.broadcast_to(v, shape + (3,)).copy()
+    try:
+        out = np.stack([np.broadcast_to(np.asarray(c, dtype=np.float64), shape)
+                        for c in g(P[..., 0], P[..., 1], P[..., 2])], axis=-1)
+        if out.shape == shape + (3,):
+            return out
+    except Exception:
+        pass
+    flat = P.reshape(-1, 3)
+    return np.array([np.asarray(g(*x), dtype=np.float64).reshape(3) for x in flat]).reshape(shape + (3,))
+
+
+def _body_load(problem: Problem, name: str, comps, gauss=None):
+    """Body force on volume group `name` (components numbers or functions)."""
+    dev = problem.device()
+    tab = _rule(problem, gauss)
+    elems = _volume_elems(problem, name)
+    if any(callable(c) for c in comps):
+        P = dev.volume_points(tab, elems)
+        G = np.stack([_eval_at(c, P) for c in comps], axis=-1)
+        return dev.body(tab, g_gp=G, elems=elems)
+    return dev.body(tab, g=[float(c) for c in comps], elems=elems)
+
+
+def _face_rule(fs):
+    name, dim, order, npe, family = refelem.element_properties(fs.etype)
+    return refelem.rule_tables(fs.etype, 2 * order + 1)
+
+
 def loadVector(problem: Problem, loads) -> VectorField | None:
-    """Traction loads load(name; fx, fy, fz) (src/linear.jl:3088-3434); each
-    component a number or a function of (x, y, z), evaluated at the face
-    Gauss points the library returns."""
+    """loadVector(problem, loads) (src/linear.jl:3088-3434) for a 3D solid:
+    on a surface group, traction load(name; fx, fy, fz) or pressure
+    load(name; p) (src/linear.jl:3196-3224, p replaces fx, fy, fz); on a
+    volume group, a body force (src/linear.jl:3054-3068).  Components and p
+    are numbers or functions of (x, y, z): traction and body-force
+    functions are evaluated at the Gauss points the library returns, p at
+    the nodes (scalarField, src/general.jl:4477-4515)."""
     if problem.type == "dummy":
         return None
     if not isinstance(loads, (list, tuple)):
@@ -450,22 +512,65 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
         if ld.problem is not None and ld.problem is not problem:
             continue
         comps = [0.0, 0.0, 0.0]
+        p = None
         for k, v in ld.values.items():
-            if k not in ("fx", "fy", "fz"):
+            if k == "p":
+                p = v
+            elif k in ("fx", "fy", "fz"):
+                comps["xyz".index(k[1])] = v
+            else:
                 raise NotImplementedError(f"loadVector: load key {k} is not on the GPU path")
-            comps["xyz".index(k[1])] = v
-        if ld.phName not in problem.mesh.faces:
-            raise KeyError(f"Physical group '{ld.phName}' not found.")
-        fs = problem.mesh.faces[ld.phName]
-        name, dim, order, npe, family = refelem.element_properties(fs.etype)
-        tab = refelem.rule_tables(fs.etype, 2 * order + 1)
-        if any(callable(c) for c in comps):
-            P = dev.face_points(fs.etype, fs.conn, tab)
-            T = np.stack([_eval_at(c, P) for c in comps], axis=-1)
-            f += dev.traction_gp(fs.etype, fs.conn, tab, T)
-        else:
-            f += dev.traction(fs.etype, fs.conn, tab, np.array([float(c) for c in comps]))
+        name = ld.phName
+        if name in problem.mesh.faces:
+            fs = problem.mesh.faces[name]
+            tab = _face_rule(fs)
+            if p is not None:
+                dNn = refelem.basis(fs.etype, refelem.node_coords(fs.etype))[1]
+                if callable(p):
+                    x = problem.mesh.coords
+                    pn = np.zeros(problem.non)
+                    nodes = np.unique(fs.conn)
+                    pn[nodes] = _eval_at(p, x[nodes][None, :, :])[0]
+                    f += dev.pressure(fs.etype, fs.conn, tab, dNn, p_nodal=pn)
+                else:
+                    f += dev.pressure(fs.etype, fs.conn, tab, dNn, p=float(p))
+            elif any(callable(c) for c in comps):
+                P = dev.face_points(fs.etype, fs.conn, tab)
+                T = np.stack([_eval_at(c, P) for c in comps], axis=-1)
+                f += dev.traction_gp(fs.etype, fs.conn, tab, T)
+            else:
+                f += dev.traction(fs.etype, fs.conn, tab, np.array([float(c) for c in comps]))
+            continue
+        if p is not None:
+            _volume_elems(problem, name)
+            raise ValueError("loadVector: pressure can be given on a surface of a 3D solid.")
+        f += _body_load(problem, name, comps)
     return VectorField(f.reshape(-1, 1), problem)
+
+
+def _linear_form(P: Problem, g, gauss=None, Γ=None, Ω=None) -> VectorField:
+    """∫(Pu ⋅ g; Ω / Γ) (assemble_linear, src/multifield.jl:3496-3748):
+    N_a g det J w over the volume groups of the problem's materials (or Ω),
+    or N_a g |t1 x t2| w over the surface group Γ; rule Gauss(2 order + 1)."""
+    dev = P.device()
+    f = np.zeros(P.ndofs())
+    if Γ is not None:
+        if Γ not in P.mesh.faces:
+            raise KeyError(f"Physical group '{Γ}' not found.")
+        fs = P.mesh.faces[Γ]
+        tab = _face_rule(fs)
+        Pq = dev.face_points(fs.etype, fs.conn, tab)
+        return VectorField((f + dev.traction_gp(fs.etype, fs.conn, tab, _eval_vec_at(g, Pq))).reshape(-1, 1), P)
+    names = [Ω] if Ω is not None else [m.phName for m in P.material]
+    tab = _rule(P, gauss)
+    for name in names:
+        elems = _volume_elems(P, name)
+        if callable(g):
+            Pq = dev.volume_points(tab, elems)
+            f += dev.body(tab, g_gp=_eval_vec_at(g, Pq), elems=elems)
+        else:
+            f += dev.body(tab, g=_eval_vec_at(g, np.zeros((1, 1, 3)))[0, 0], elems=elems)
+    return VectorField(f.reshape(-1, 1), P)
 
 
 def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=False,

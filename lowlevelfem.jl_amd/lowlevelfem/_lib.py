@@ -22,6 +22,7 @@ EXPORTED = [
     "llfem_ctx_create", "llfem_ctx_destroy", "llfem_ctx_synchronize", "llfem_ctx_bytes_in_use",
     "llfem_last_error", "llfem_mesh_create", "llfem_mesh_destroy", "llfem_mesh_pattern",
     "llfem_assemble", "llfem_load_traction", "llfem_face_points", "llfem_load_traction_gp",
+    "llfem_load_body", "llfem_volume_points", "llfem_load_pressure",
     "llfem_solve", "llfem_solve_csc", "llfem_solve_csc_noalloc",
     "llfem_assemble_batch", "llfem_mesh_values", "llfem_autotune",
     "llfem_server_create", "llfem_server_submit", "llfem_server_wait", "llfem_server_destroy",
@@ -101,6 +102,9 @@ def load_library(path: str = LIB_PATH):
     lib.llfem_load_traction.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, P]
     lib.llfem_face_points.argtypes = [P, I32, I32, P, I64, I32, I32, P, P]
     lib.llfem_load_traction_gp.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, P]
+    lib.llfem_load_body.argtypes = [P, I32, P, P, P, P, I64, P, P, P]
+    lib.llfem_volume_points.argtypes = [P, I32, P, P, I64, P]
+    lib.llfem_load_pressure.argtypes = [P, I32, I32, P, I64, I32, I32, P, P, P, P, D, P, P]
     lib.llfem_solve.argtypes = [P, P, P, I64, I32, P, I32, D, I64, P, C.POINTER(Stats)]
     lib.llfem_solve_csc.argtypes = [P, I64, P, P, P, I32, P, P, I64, P, I32, D, I64, P,
                                     C.POINTER(Stats)]
@@ -294,6 +298,51 @@ class DeviceMesh:
         dN = np.ascontiguousarray(tables.dN)
         check(self.ctx.lib.llfem_load_traction_gp(self.h, int(face_type), fc.shape[1], ptr(fc), fc.shape[0], 0,
                                                   tables.q, ptr(w), ptr(Nr), ptr(dN), ptr(tg), ptr(f)))
+        return f
+
+    def _elems(self, elems):
+        return (None, 0) if elems is None else (np.ascontiguousarray(elems, dtype=np.int64),
+                                                len(elems))
+
+    def body(self, tables, g=None, g_gp=None, elems=None):
+        """Body-force load vector over the elements `elems` (0-based, None =
+        all): constant g [3], or g_gp (ne, q, 3) per Gauss point."""
+        el, ne = self._elems(elems)
+        w, Nr, dN = (np.ascontiguousarray(a, dtype=np.float64) for a in (tables.weights, tables.N, tables.dN))
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        gp = None
+        if g_gp is not None:
+            gp = np.ascontiguousarray(g_gp, dtype=np.float64)
+            nset = self.E if el is None else ne
+            if gp.shape != (nset, tables.q, 3):
+                raise ValueError(f"body: g_gp has shape {gp.shape}, expected {(nset, tables.q, 3)}")
+        f = np.empty(3 * self.N, dtype=np.float64)
+        check(self.ctx.lib.llfem_load_body(self.h, tables.q, ptr(w), ptr(Nr), ptr(dN), ptr(el), ne, ptr(gg),
+                                           ptr(gp), ptr(f)))
+        return f
+
+    def volume_points(self, tables, elems=None):
+        """Physical coordinates of the volume Gauss points, (ne, q, 3)."""
+        el, ne = self._elems(elems)
+        Nr = np.ascontiguousarray(tables.N, dtype=np.float64)
+        out = np.empty((self.E if el is None else ne, tables.q, 3), dtype=np.float64)
+        check(self.ctx.lib.llfem_volume_points(self.h, tables.q, ptr(Nr), ptr(el), ne, ptr(out)))
+        return out
+
+    def pressure(self, face_type, face_conn, tables, dN_nodes, p=0.0, p_nodal=None):
+        """Load vector of a pressure on a face set: constant p, or p_nodal [N]."""
+        fc = np.ascontiguousarray(face_conn, dtype=np.int64)
+        w, Nr, dN = (np.ascontiguousarray(a, dtype=np.float64) for a in (tables.weights, tables.N, tables.dN))
+        dNn = np.ascontiguousarray(dN_nodes, dtype=np.float64)
+        if dNn.shape != (fc.shape[1], fc.shape[1], 3):
+            raise ValueError(f"pressure: dN_nodes has shape {dNn.shape}, expected {(fc.shape[1], fc.shape[1], 3)}")
+        pn = None if p_nodal is None else np.ascontiguousarray(p_nodal, dtype=np.float64)
+        if pn is not None and pn.size != self.N:
+            raise ValueError(f"pressure: p_nodal has {pn.size} values, the mesh {self.N} nodes")
+        f = np.empty(3 * self.N, dtype=np.float64)
+        check(self.ctx.lib.llfem_load_pressure(self.h, int(face_type), fc.shape[1], ptr(fc), fc.shape[0], 0,
+                                               tables.q, ptr(w), ptr(Nr), ptr(dN), ptr(dNn), float(p), ptr(pn),
+                                               ptr(f)))
         return f
 
     def solve(self, f, fixed, ufix=None, method=PCG_JACOBI_FP64, reltol=1e-10, maxiter=100000):

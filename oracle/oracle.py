@@ -191,6 +191,85 @@ def traction_fn(mesh, faceset, tab, fns):
     return f, T
 
 
+def volume_points(mesh, tab, elems=None):
+    """Physical Gauss-point coordinates of volume elements, (ne, q, 3):
+    x = h[:, j]' * ncoord (src/linear.jl:3278-3280)."""
+    conn = mesh.conn if elems is None else mesh.conn[np.asarray(elems, dtype=np.int64)]
+    return np.einsum("qa,eai->eqi", tab.N, mesh.coords[conn])
+
+
+def body_force(mesh, tab, g=None, g_gp=None, elems=None):
+    """loadVector on a volume group, the "Volume -> body force" case
+    (src/linear.jl:3054-3068): per element and Gauss point f from
+    _loadvec_helper (a constant g, or g_gp [ne][q][3] at volume_points),
+    Ja = jacDet[j] (det J, DIM == dim == 3, src/linear.jl:3374-3375),
+    f1 += H1' f Ja w, fp[nn2] += f1 in element order (src/linear.jl:3404-3413).
+    The same integral is ∫(Pu ⋅ g; Ω) (assemble_element_vector!,
+    src/multifield.jl:3386-3462).  Plain numpy loops (small meshes)."""
+    elist = np.arange(mesh.E) if elems is None else np.asarray(elems, dtype=np.int64)
+    f = np.zeros(3 * mesh.N)
+    for k, e in enumerate(elist):
+        nodes = mesh.conn[e]
+        X = mesh.coords[nodes]
+        f1 = np.zeros((len(nodes), 3))
+        for j in range(tab.q):
+            J = X.T @ tab.dN[j]                        # J[i][d] = sum_b x_b,i dN_b,d
+            Ja = np.linalg.det(J)
+            gq = np.asarray(g, dtype=np.float64) if g_gp is None else g_gp[k, j]
+            f1 += np.outer(tab.N[j], gq) * Ja * tab.weights[j]
+        for a, v in enumerate(nodes):
+            f[3 * v:3 * v + 3] += f1[a]
+    return f
+
+
+def face_node_normals(mesh, faceset, dN_nodes):
+    """normalVector (src/general.jl:4848-4940): per face element and local
+    node k, the unit normal (∂x/∂ξ × ∂x/∂η)/|…| with the derivatives of the
+    face map at the face's own reference node k.  (F, nf, 3)."""
+    X = mesh.coords[faceset.conn]                      # (F, nf, 3)
+    t1 = np.einsum("ka,fai->fki", dN_nodes[:, :, 0], X)
+    t2 = np.einsum("ka,fai->fki", dN_nodes[:, :, 1], X)
+    n = np.cross(t1, t2)
+    return n / np.linalg.norm(n, axis=-1, keepdims=True)
+
+
+def pressure(mesh, faceset, tab, dN_nodes, p=0.0, p_nodal=None):
+    """loadVector(problem, [load(name, p=p)]) (src/linear.jl:3196-3224):
+    nv = -normalVector, the nodal traction t = elementsToNodes((nv ⋅ e_c) p)
+    -- per node the sum over the face elements at it of -n_c p, times
+    1/count (src/general.jl:5519-5640) -- then the traction branch with a
+    nodal field, t(x_q) = h[:, j]' t[nnet] (_loadvec_helper,
+    src/linear.jl:3033-3044), f1 += H1' t Ja w (:3404-3413).  p is a number
+    or p_nodal [N] (a function's values at the nodes, scalarField,
+    src/general.jl:4477-4515)."""
+    nrm = face_node_normals(mesh, faceset, dN_nodes)
+    F, nf = faceset.conn.shape
+    acc = np.zeros((mesh.N, 3))
+    cnt = np.zeros(mesh.N, dtype=np.int64)
+    for e in range(F):
+        for k, v in enumerate(faceset.conn[e]):
+            pv = p if p_nodal is None else p_nodal[v]
+            acc[v] += (-nrm[e, k]) * pv
+            cnt[v] += 1
+    tn = np.zeros((mesh.N, 3))
+    hit = cnt > 0
+    tn[hit] = acc[hit] * (1.0 / cnt[hit])[:, None]
+    f = np.zeros(3 * mesh.N)
+    for e in range(F):
+        nodes = faceset.conn[e]
+        X = mesh.coords[nodes]
+        f1 = np.zeros((nf, 3))
+        for j in range(tab.q):
+            t1 = tab.dN[j, :, 0] @ X
+            t2 = tab.dN[j, :, 1] @ X
+            Ja = np.linalg.norm(np.cross(t1, t2))
+            tq = tab.N[j] @ tn[nodes]
+            f1 += np.outer(tab.N[j], tq) * Ja * tab.weights[j]
+        for a, v in enumerate(nodes):
+            f[3 * v:3 * v + 3] += f1[a]
+    return f, tn
+
+
 def pcg(K: sp.csc_matrix, f, fixed, ufix=None, reltol=1e-10, maxiter=100000):
     """solveDisplacement(K, f; support, iterative=true, Pl=Jacobi)
     (src/linear.jl:3711-3743) with IterativeSolvers.cg semantics."""
