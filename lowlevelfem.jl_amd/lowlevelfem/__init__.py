@@ -145,17 +145,23 @@ class Problem:
         self.name = self.mesh.name if self.mesh is not None else "dummy"
         self.non = self.mesh.N if self.mesh is not None else 0
         self._ctx = context
-        self._dev = None
+        self._devs = None
 
     def ndofs(self):
         return self.pdim * self.non
 
     def device(self) -> _lib.DeviceMesh:
-        """The device-resident mesh/pattern (built once, reused)."""
-        if self._dev is None:
+        """The device-resident mesh/pattern of the first element block."""
+        return self.devices()[0][1]
+
+    def devices(self) -> list:
+        """(Block, DeviceMesh) per volume element type (built once, reused);
+        every device mesh holds all node coordinates."""
+        if self._devs is None:
             ctx = self._ctx or _lib.default_context()
-            self._dev = _lib.DeviceMesh(ctx, self.mesh.coords, self.mesh.etype, self.mesh.conn)
-        return self._dev
+            self._devs = [(b, _lib.DeviceMesh(ctx, self.mesh.coords, b.etype, b.conn))
+                          for b in self.mesh.element_blocks()]
+        return self._devs
 
 
 @dataclass
@@ -259,41 +265,62 @@ def _csc(n, colptr, rowval, val):
 _PREC = {"fp64": _lib.FP64, "fp32": _lib.FP32_INT_FP64_SUM, "bf16x3": _lib.BF16X3}
 
 
-def _rule(problem: Problem, gauss=None):
-    name, dim, order, npe, family = refelem.element_properties(problem.mesh.etype)
-    return refelem.rule_tables(problem.mesh.etype, refelem.gauss_degree(order, gauss))
+def _rule(problem: Problem, gauss=None, etype=None):
+    et = problem.mesh.etype if etype is None else etype
+    name, dim, order, npe, family = refelem.element_properties(et)
+    return refelem.rule_tables(et, refelem.gauss_degree(order, gauss))
+
+
+def _sum_parts(parts):
+    """Operators of the element blocks added in block order (the reference
+    appends every element type's triplets before one sparse(),
+    src/linear.jl:225-321)."""
+    A = parts[0]
+    for P in parts[1:]:
+        A = A + P
+    return A.tocsc()
 
 
 def build_csc_pattern(Pu: Problem, Ps: Problem | None = None) -> sp.csc_matrix:
-    """Structural CSC pattern with zero values (src/multifield.jl:2083-2250)."""
+    """Structural CSC pattern with zero values (src/multifield.jl:2083-2250);
+    with several element types, the union of their patterns."""
     if Ps is not None and Ps.mesh is not Pu.mesh:
         raise ValueError("Both problems must refer to the same gmsh model/mesh.")
-    dev = Pu.device()
-    colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
-    return _csc(Pu.ndofs(), colptr, rowval, np.zeros(rowval.size))
+    parts = []
+    for _, dev in Pu.devices():
+        colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
+        parts.append(_csc(Pu.ndofs(), colptr, rowval, np.ones(rowval.size)))
+    A = _sum_parts(parts)
+    A.data[:] = 0.0
+    return A
 
 
-def _material_groups(problem: Problem):
-    """(material, element scale or None) per material group.  Each material
-    names a volume physical group (a generated box has the one group "body");
-    a single material whose group holds every element runs the plain
-    assembly, otherwise each group's elements get scale 1 and the others 0
-    (the reference loops over problem.material and their groups' elements,
-    src/linear.jl:190-330)."""
-    vols = problem.mesh.volumes
+def _material_groups(problem: Problem, blk):
+    """(material, element scale or None) per material group that has
+    elements in element block `blk`.  Each material names a volume physical
+    group (a generated box has the one group "body"); a single material
+    whose group holds every element of the block runs the plain assembly,
+    otherwise each group's elements get scale 1 and the others 0 (the
+    reference loops over problem.material, then the group's element types
+    and elements, src/linear.jl:190-330)."""
+    vols = blk.volumes
+    E = blk.conn.shape[0]
+    known = set().union(*[b.volumes.keys() for b in problem.mesh.element_blocks()])
     if len(problem.material) == 1:
         name = problem.material[0].phName
-        if name in vols and len(np.unique(vols[name])) == problem.mesh.E:
+        if name in vols and len(np.unique(vols[name])) == E:
             return [(problem.material[0], None)]
-        if name not in vols:
-            if vols or name != "body":      # a generated box has the one volume group "body"
+        if name not in known:
+            if known or name != "body":     # a generated box has the one volume group "body"
                 raise KeyError(f"Physical group '{name}' not found.")
             return [(problem.material[0], None)]
     out = []
     for mat in problem.material:
-        if mat.phName not in vols:
+        if mat.phName not in known:
             raise KeyError(f"Physical group '{mat.phName}' not found.")
-        sc = np.zeros(problem.mesh.E)
+        if mat.phName not in vols:
+            continue                        # no elements of this type in the group
+        sc = np.zeros(E)
         sc[vols[mat.phName]] = 1.0
         out.append((mat, sc))
     return out
@@ -308,26 +335,32 @@ def stiffnessMatrix(problem: Problem, elements=(), forceOneThread=True, gauss=No
     to the groups before it (accumulate), so the device copy is the full K."""
     if problem.type == "dummy":
         return None
-    dev = problem.device()
-    tab = _rule(problem, gauss)
-    groups = _material_groups(problem)
-    last = len(groups) - 1
-    for i, (m, sc) in enumerate(groups):
-        val, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision],
-                               element_scale=sc, accumulate=i > 0, copy_out=i == last)
-    colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
-    A = _csc(problem.ndofs(), colptr, rowval, val)
+    parts, st = [], None
+    for blk, dev in problem.devices():
+        tab = _rule(problem, gauss, blk.etype)
+        groups = _material_groups(problem, blk)
+        last = len(groups) - 1
+        for i, (m, sc) in enumerate(groups):
+            val, st = dev.assemble(_lib.OP_STIFFNESS, tab, E=m.E, nu=m.nu, precision=_PREC[precision],
+                                   element_scale=sc, accumulate=i > 0, copy_out=i == last)
+        if groups:
+            colptr, rowval = dev.pattern(_lib.OP_STIFFNESS)
+            parts.append(_csc(problem.ndofs(), colptr, rowval, val))
+    if not parts:
+        raise ValueError("stiffnessMatrix: no material group has elements")
+    A = _sum_parts(parts)
     if dropzeros_:
         A = dropzeros(A)
-    return SystemMatrix(A, problem, _device_k(problem, st))
+    return SystemMatrix(A, problem, _device_k(problem, st, resident=len(problem.devices()) == 1))
 
 
-def _device_k(problem: Problem, st) -> dict:
+def _device_k(problem: Problem, st, resident=True) -> dict:
     """Stats of a stiffness assembly, tagged with the device copy it left on
-    the problem's mesh (solveDisplacement's preconditioner="amg" solves that copy)."""
+    the problem's mesh (solveDisplacement's preconditioner="amg" solves that
+    copy; a mesh of several element types has no single device copy)."""
     problem._k_version = getattr(problem, "_k_version", 0) + 1
     d = st.as_dict()
-    d["device_k_version"] = problem._k_version
+    d["device_k_version"] = problem._k_version if resident else None
     return d
 
 
@@ -359,6 +392,8 @@ def _integral_id(a, c, b, gauss, csc_matrix):
         raise NotImplementedError("∫(Id ⋅ C ⋅ Id): only a scalar coefficient is on the GPU path")
     rho = float(np.asarray(c, dtype=np.float64).reshape(-1)[0])
     P = a.P
+    if len(P.devices()) > 1:
+        raise NotImplementedError("∫(Id ⋅ c ⋅ Id) on a mesh of several element types is not on the GPU path")
     n = P.ndofs()
     dev = P.device()
     val, st = dev.assemble(_lib.OP_MASS, _rule(P, gauss), rho=rho, lumped=False)
@@ -398,6 +433,16 @@ def integral(a, D, b=None, gauss=None, csc_matrix=None, precision="fp64", thread
         raise NotImplementedError("only ∫(SymGrad(Pu) ⋅ D ⋅ SymGrad(Pu)) is on the GPU path")
     P = a.P
     D = np.asarray(D, dtype=np.float64).reshape(6, 6)
+    if len(P.devices()) > 1:
+        if csc_matrix is not None:
+            raise NotImplementedError("∫(SymGrad ⋅ D ⋅ SymGrad; csc_matrix) on a mesh of several element "
+                                      "types is not on the GPU path")
+        parts = []
+        for blk, dv in P.devices():
+            v, st = dv.assemble(_lib.OP_STIFFNESS, _rule(P, gauss, blk.etype), D=D, precision=_PREC[precision])
+            cp, rv = dv.pattern(_lib.OP_STIFFNESS)
+            parts.append(_csc(P.ndofs(), cp, rv, v))
+        return SystemMatrix(dropzeros(_sum_parts(parts)), P, _device_k(P, st, resident=False))
     dev = P.device()
     tab = _rule(P, gauss)
     val, st = dev.assemble(_lib.OP_STIFFNESS, tab, D=D, precision=_PREC[precision])
@@ -415,18 +460,24 @@ def massMatrix(problem: Problem, lumped=True, gauss=None) -> SystemMatrix | None
     """massMatrix(problem; lumped=true) (src/linear.jl:1843-2004)."""
     if problem.type == "dummy":
         return None
-    dev = problem.device()
-    tab = _rule(problem, gauss)
-    val = None
-    for m, sc in _material_groups(problem):
-        v, st = dev.assemble(_lib.OP_MASS, tab, rho=m.rho, lumped=lumped, element_scale=sc)
-        val = v if val is None else val + v
     n = problem.ndofs()
-    if lumped:
-        A = sp.diags(val, 0, shape=(n, n), format="csc")
-        return SystemMatrix(dropzeros(A), problem, st.as_dict())
-    colptr, rowval = dev.pattern(_lib.OP_MASS)
-    return SystemMatrix(dropzeros(_csc(n, colptr, rowval, val)), problem, st.as_dict())
+    parts, st = [], None
+    for blk, dev in problem.devices():
+        tab = _rule(problem, gauss, blk.etype)
+        val = None
+        for m, sc in _material_groups(problem, blk):
+            v, st = dev.assemble(_lib.OP_MASS, tab, rho=m.rho, lumped=lumped, element_scale=sc)
+            val = v if val is None else val + v
+        if val is None:
+            continue
+        if lumped:
+            parts.append(sp.diags(val, 0, shape=(n, n), format="csc"))
+        else:
+            colptr, rowval = dev.pattern(_lib.OP_MASS)
+            parts.append(_csc(n, colptr, rowval, val))
+    if not parts:
+        raise ValueError("massMatrix: no material group has elements")
+    return SystemMatrix(dropzeros(_sum_parts(parts)), problem, st.as_dict())
 
 
 def _eval_at(fn, P):
@@ -444,13 +495,16 @@ def _eval_at(fn, P):
         return np.array([float(fn(*x)) for x in flat]).reshape(shape)
 
 
-def _volume_elems(problem: Problem, name: str):
-    """Element indices of volume group `name` (None: every element, the
-    generated box's one group "body")."""
-    vols = problem.mesh.volumes
-    if name in vols:
-        return np.asarray(vols[name], dtype=np.int64)
-    if name == "body" and not vols:
+def _volume_elems(problem: Problem, blk, name: str):
+    """Element indices of volume group `name` in element block `blk` (None:
+    every element, the generated box's one group "body"; an empty array: the
+    group has no elements of this type)."""
+    blocks = problem.mesh.element_blocks()
+    if blk.volumes.get(name) is not None:
+        return np.asarray(blk.volumes[name], dtype=np.int64)
+    if any(name in b.volumes for b in blocks):
+        return np.zeros(0, dtype=np.int64)
+    if name == "body" and not any(b.volumes for b in blocks):
         return None
     raise KeyError(f"Physical group '{name}' not found.")
 
@@ -478,15 +532,21 @@ def _eval_vec_at(g, P):
 
 
 def _body_load(problem: Problem, name: str, comps, gauss=None):
-    """Body force on volume group `name` (components numbers or functions)."""
-    dev = problem.device()
-    tab = _rule(problem, gauss)
-    elems = _volume_elems(problem, name)
-    if any(callable(c) for c in comps):
-        P = dev.volume_points(tab, elems)
-        G = np.stack([_eval_at(c, P) for c in comps], axis=-1)
-        return dev.body(tab, g_gp=G, elems=elems)
-    return dev.body(tab, g=[float(c) for c in comps], elems=elems)
+    """Body force on volume group `name` (components numbers or functions),
+    summed over the element blocks."""
+    f = np.zeros(problem.ndofs())
+    for blk, dev in problem.devices():
+        elems = _volume_elems(problem, blk, name)
+        if elems is not None and elems.size == 0:
+            continue
+        tab = _rule(problem, gauss, blk.etype)
+        if any(callable(c) for c in comps):
+            P = dev.volume_points(tab, elems)
+            G = np.stack([_eval_at(c, P) for c in comps], axis=-1)
+            f += dev.body(tab, g_gp=G, elems=elems)
+        else:
+            f += dev.body(tab, g=[float(c) for c in comps], elems=elems)
+    return f
 
 
 def _face_rule(fs):
@@ -521,8 +581,7 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
             else:
                 raise NotImplementedError(f"loadVector: load key {k} is not on the GPU path")
         name = ld.phName
-        if name in problem.mesh.faces:
-            fs = problem.mesh.faces[name]
+        for fs in (problem.mesh.face_sets(name) if name in problem.mesh.faces else []):
             tab = _face_rule(fs)
             if p is not None:
                 dNn = refelem.basis(fs.etype, refelem.node_coords(fs.etype))[1]
@@ -540,9 +599,10 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
                 f += dev.traction_gp(fs.etype, fs.conn, tab, T)
             else:
                 f += dev.traction(fs.etype, fs.conn, tab, np.array([float(c) for c in comps]))
+        if name in problem.mesh.faces:
             continue
         if p is not None:
-            _volume_elems(problem, name)
+            _volume_elems(problem, problem.mesh.element_blocks()[0], name)
             raise ValueError("loadVector: pressure can be given on a surface of a 3D solid.")
         f += _body_load(problem, name, comps)
     return VectorField(f.reshape(-1, 1), problem)
@@ -555,21 +615,23 @@ def _linear_form(P: Problem, g, gauss=None, Γ=None, Ω=None) -> VectorField:
     dev = P.device()
     f = np.zeros(P.ndofs())
     if Γ is not None:
-        if Γ not in P.mesh.faces:
-            raise KeyError(f"Physical group '{Γ}' not found.")
-        fs = P.mesh.faces[Γ]
-        tab = _face_rule(fs)
-        Pq = dev.face_points(fs.etype, fs.conn, tab)
-        return VectorField((f + dev.traction_gp(fs.etype, fs.conn, tab, _eval_vec_at(g, Pq))).reshape(-1, 1), P)
+        for fs in P.mesh.face_sets(Γ):
+            tab = _face_rule(fs)
+            Pq = dev.face_points(fs.etype, fs.conn, tab)
+            f += dev.traction_gp(fs.etype, fs.conn, tab, _eval_vec_at(g, Pq))
+        return VectorField(f.reshape(-1, 1), P)
     names = [Ω] if Ω is not None else [m.phName for m in P.material]
-    tab = _rule(P, gauss)
     for name in names:
-        elems = _volume_elems(P, name)
-        if callable(g):
-            Pq = dev.volume_points(tab, elems)
-            f += dev.body(tab, g_gp=_eval_vec_at(g, Pq), elems=elems)
-        else:
-            f += dev.body(tab, g=_eval_vec_at(g, np.zeros((1, 1, 3)))[0, 0], elems=elems)
+        for blk, dv in P.devices():
+            elems = _volume_elems(P, blk, name)
+            if elems is not None and elems.size == 0:
+                continue
+            tab = _rule(P, gauss, blk.etype)
+            if callable(g):
+                Pq = dv.volume_points(tab, elems)
+                f += dv.body(tab, g_gp=_eval_vec_at(g, Pq), elems=elems)
+            else:
+                f += dv.body(tab, g=_eval_vec_at(g, np.zeros((1, 1, 3)))[0, 0], elems=elems)
     return VectorField(f.reshape(-1, 1), P)
 
 
@@ -605,6 +667,8 @@ def solveDisplacement(K: SystemMatrix, f: VectorField, support=(), iterative=Fal
     meth = _lib.PCG_MIXED_REFINE if method == "mixed" else _lib.PCG_JACOBI_FP64
     dev = problem.device()
     if preconditioner == "amg":
+        if len(problem.devices()) > 1:
+            raise NotImplementedError("solveDisplacement: preconditioner=\"amg\" needs a mesh of one element type")
         if K.stats.get("device_k_version") is None or \
                 K.stats["device_k_version"] != getattr(problem, "_k_version", None):
             raise ValueError("solveDisplacement: preconditioner=\"amg\" solves the stiffness kept on the "

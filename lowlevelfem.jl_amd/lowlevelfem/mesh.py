@@ -36,13 +36,34 @@ class FaceSet:
 
 
 @dataclass
+class Block:
+    """Volume elements of one gmsh element type."""
+    etype: int
+    conn: np.ndarray           # (E, npe) int64, 0-based node ids
+    volumes: dict = field(default_factory=dict)  # volume physical group -> element indices in this block
+
+
+@dataclass
 class Mesh:
     coords: np.ndarray         # (N, 3) float64
-    etype: int                 # gmsh volume element type
+    etype: int                 # gmsh volume element type (of the first block)
     conn: np.ndarray           # (E, npe) int64, 0-based node ids
-    faces: dict = field(default_factory=dict)   # name -> FaceSet
+    faces: dict = field(default_factory=dict)   # name -> FaceSet, or a list of FaceSets of several types
     name: str = "structured_box"
     volumes: dict = field(default_factory=dict)  # volume physical group -> element indices
+    blocks: list = field(default_factory=list)   # further volume element types: Block per type
+
+    def element_blocks(self) -> list:
+        """Every volume element block: (etype, conn, volumes) first, then the
+        further element types in file order."""
+        return [Block(self.etype, self.conn, self.volumes)] + list(self.blocks)
+
+    def face_sets(self, name: str) -> list:
+        """The face sets of 2D physical group `name`, one per face element type."""
+        if name not in self.faces:
+            raise KeyError(f"Physical group '{name}' not found.")
+        fs = self.faces[name]
+        return list(fs) if isinstance(fs, (list, tuple)) else [fs]
 
     @property
     def N(self) -> int:
@@ -59,13 +80,13 @@ class Mesh:
     def nodes_on(self, group: str) -> np.ndarray:
         """Sorted unique node ids (0-based) of a physical group; the
         reference's nodesOnPhysicalGroup (src/general.jl:5249-5275)."""
-        if group in self.volumes:
-            return np.unique(self.conn[self.volumes[group]])
+        blocks = self.element_blocks()
+        if any(group in b.volumes for b in blocks):
+            return np.unique(np.concatenate([b.conn[b.volumes[group]].ravel() for b in blocks
+                                             if group in b.volumes]))
         if group == "body":
             return np.arange(self.N, dtype=np.int64)
-        if group not in self.faces:
-            raise KeyError(f"Physical group '{group}' not found.")
-        return np.unique(self.faces[group].conn)
+        return np.unique(np.concatenate([fs.conn.ravel() for fs in self.face_sets(group)]))
 
 
 def _cells(lx, ly, lz, n, dx, dy, dz):

@@ -10,14 +10,16 @@ This module reads the published MSH 4.1 ASCII layout directly -- sections
 same ``Mesh`` the structured generators build: node ``tag - 1`` is row
 ``tag - 1`` of the coordinates (gmsh's own numbering, so the CSC pattern is
 the one the reference builds on this file), the volume elements in file order
-with gmsh's node order, the element indices of every 3D physical group (the
-material groups), and the elements of every 2D physical group as a
-``FaceSet`` under the group's name.  Files ending in ``.gz`` are read through
+with gmsh's node order -- one block per volume element type, types in order
+of first appearance -- the element indices of every 3D physical group per
+block (the material groups), and the elements of every 2D physical group as
+a ``FaceSet`` (a list of them when the group has several face types) under
+the group's name.  Files ending in ``.gz`` are read through
 gzip.
 
 Supported element types are those whose node order here is gmsh's (hex8,
 hex27, tet4, tet10, prism6 volumes; tri3, tri6, quad4, quad9 faces).  Binary
-MSH files, MSH 2.x and meshes mixing volume element types are rejected.
+MSH files and MSH 2.x are rejected.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ import os
 import numpy as np
 
 from . import refelem as re
-from .mesh import FaceSet, Mesh
+from .mesh import Block, FaceSet, Mesh
 
 # element types whose node order in refelem is gmsh's documented order
 GMSH_ORDERED = {2, 3, 4, 5, 6, 9, 10, 11, 12}
@@ -132,22 +134,29 @@ def read_msh(path) -> Mesh:
     vol = [b for b in blocks if b[0] == 3]
     if not vol:
         raise ValueError("read_msh: no volume elements")
-    vtypes = {b[2] for b in vol}
-    if len(vtypes) != 1:
-        raise NotImplementedError(f"read_msh: mixed volume element types {sorted(vtypes)}")
-    vtype = vtypes.pop()
-    if vtype not in GMSH_ORDERED or re.element_properties(vtype)[1] != 3:
-        raise NotImplementedError(f"read_msh: volume element type {vtype} is not supported")
-    conn = np.ascontiguousarray(np.concatenate([b[3] for b in vol]))
-    # volume physical groups: element indices (file order) of their entities
-    first = np.cumsum([0] + [b[3].shape[0] for b in vol])
-    volumes = {}
-    for (dim, ptag), name in names.items():
-        if dim != 3:
-            continue
-        idx = [np.arange(first[i], first[i + 1]) for i, b in enumerate(vol) if ptag in ent_phys.get((3, b[1]), [])]
-        if idx:
-            volumes[name] = np.concatenate(idx)
+    # one element block per volume element type, types in order of first
+    # appearance, elements of a type in file order
+    vtypes = list(dict.fromkeys(b[2] for b in vol))
+    for vt in vtypes:
+        if vt not in GMSH_ORDERED or re.element_properties(vt)[1] != 3:
+            raise NotImplementedError(f"read_msh: volume element type {vt} is not supported")
+    eblocks = []
+    for vt in vtypes:
+        vb = [b for b in vol if b[2] == vt]
+        conn = np.ascontiguousarray(np.concatenate([b[3] for b in vb]))
+        if conn.min() < 0 or conn.max() >= nnodes:
+            raise ValueError("read_msh: element node tag out of range")
+        # volume physical groups: element indices (file order within the type) of their entities
+        first = np.cumsum([0] + [b[3].shape[0] for b in vb])
+        volumes = {}
+        for (dim, ptag), name in names.items():
+            if dim != 3:
+                continue
+            idx = [np.arange(first[i], first[i + 1]) for i, b in enumerate(vb)
+                   if ptag in ent_phys.get((3, b[1]), [])]
+            if idx:
+                volumes[name] = np.concatenate(idx)
+        eblocks.append(Block(vt, conn, volumes))
 
     faces = {}
     for (dim, ptag), name in names.items():
@@ -156,11 +165,12 @@ def read_msh(path) -> Mesh:
         fb = [b for b in blocks if b[0] == 2 and ptag in ent_phys.get((2, b[1]), [])]
         if not fb:
             continue
-        ftypes = {b[2] for b in fb}
-        if len(ftypes) != 1 or next(iter(ftypes)) not in GMSH_ORDERED:
-            raise NotImplementedError(f"read_msh: face group '{name}' has element types {sorted(ftypes)}")
-        faces[name] = FaceSet(etype=next(iter(ftypes)), conn=np.ascontiguousarray(np.concatenate([b[3] for b in fb])))
-    if conn.min() < 0 or conn.max() >= nnodes:
-        raise ValueError("read_msh: element node tag out of range")
-    return Mesh(coords=coords, etype=vtype, conn=conn, faces=faces,
-                name=os.path.basename(str(path)).split(".")[0], volumes=volumes)
+        sets = []
+        for ft in dict.fromkeys(b[2] for b in fb):
+            if ft not in GMSH_ORDERED:
+                raise NotImplementedError(f"read_msh: face group '{name}' has element type {ft}")
+            sets.append(FaceSet(etype=ft, conn=np.ascontiguousarray(np.concatenate([b[3] for b in fb if b[2] == ft]))))
+        faces[name] = sets[0] if len(sets) == 1 else sets
+    b0 = eblocks[0]
+    return Mesh(coords=coords, etype=b0.etype, conn=b0.conn, faces=faces,
+                name=os.path.basename(str(path)).split(".")[0], volumes=b0.volumes, blocks=eblocks[1:])

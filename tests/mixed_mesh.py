@@ -1,0 +1,76 @@
+"""Test helper: a box of hexahedra and prisms (one mesh, two volume element
+types) and an MSH 4.1 ASCII writer for it.
+
+The box is a structured hex8 mesh whose cells with i >= split are each cut
+into two 6-node wedges along the diagonal plane through hex nodes 0, 2, 6, 4
+(gmsh node order), so every face shared by a hexahedron and a wedge is the
+same quadrilateral.  Material groups: "steel" (cells with k < nz/2) and
+"alu" (the rest), each holding elements of both types.  Face groups: "left"
+(x = 0, hex quads), "right" (x = lx, wedge quads) and "front" (z = lz: hex
+quads and wedge triangles, two face types in one group)."""
+import numpy as np
+
+from lowlevelfem import mesh as M
+
+WEDGES = ([0, 1, 2, 4, 5, 6], [0, 2, 3, 4, 6, 7])
+
+
+def hex_prism_box(cells=(4, 3, 4), lx=1.0, ly=0.8, lz=1.2, split=None):
+    nx, ny, nz = cells
+    split = nx // 2 if split is None else split
+    m = M.structured_box_mesh(lx=lx, ly=ly, lz=lz, cells=cells)
+    c = np.arange(m.E)
+    i, k = c % nx, c // (nx * ny)
+    is_hex = i < split
+    steel_cell = k < nz // 2
+    hconn = np.ascontiguousarray(m.conn[is_hex])
+    pc = m.conn[~is_hex]
+    pconn = np.ascontiguousarray(np.stack([pc[:, WEDGES[0]], pc[:, WEDGES[1]]], axis=1).reshape(-1, 6))
+    hvol = {"steel": np.flatnonzero(steel_cell[is_hex]), "alu": np.flatnonzero(~steel_cell[is_hex])}
+    pst = np.repeat(steel_cell[~is_hex], 2)
+    pvol = {"steel": np.flatnonzero(pst), "alu": np.flatnonzero(~pst)}
+    top = k == nz - 1
+    fq = m.conn[top & is_hex][:, [4, 5, 6, 7]]
+    ft = m.conn[top & ~is_hex]
+    ftri = np.stack([ft[:, [4, 5, 6]], ft[:, [4, 6, 7]]], axis=1).reshape(-1, 3)
+    faces = {"left": m.faces["left"], "right": m.faces["right"],
+             "front": [M.FaceSet(3, np.ascontiguousarray(fq)), M.FaceSet(2, np.ascontiguousarray(ftri))]}
+    return M.Mesh(m.coords, 5, hconn, faces=faces, name="hexprism", volumes=hvol,
+                  blocks=[M.Block(6, pconn, pvol)])
+
+
+def write_msh(mesh, path):
+    """MSH 4.1 ASCII: one volume entity per (material group, element type),
+    one surface entity per (face group, face type); node tags 1..N."""
+    groups = ["steel", "alu"]
+    fnames = list(mesh.faces)
+    ptag = {("v", g): i + 1 for i, g in enumerate(groups)}
+    ptag.update({("s", f): len(groups) + i + 1 for i, f in enumerate(fnames)})
+    lo, hi = mesh.coords.min(axis=0), mesh.coords.max(axis=0)
+    bbox = " ".join(f"{v:.17g}" for v in list(lo) + list(hi))
+    vol_ents, surf_ents = [], []
+    for g in groups:
+        for b in mesh.element_blocks():
+            vol_ents.append((len(vol_ents) + 1, ptag[("v", g)], b.etype, b.conn[b.volumes[g]]))
+    for f in fnames:
+        for fs in mesh.face_sets(f):
+            surf_ents.append((len(surf_ents) + 1, ptag[("s", f)], fs.etype, fs.conn))
+    out = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(ptag))]
+    out += [f'3 {ptag[("v", g)]} "{g}"' for g in groups] + [f'2 {ptag[("s", f)]} "{f}"' for f in fnames]
+    out += ["$EndPhysicalNames", "$Entities", f"0 0 {len(surf_ents)} {len(vol_ents)}"]
+    out += [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in surf_ents] + [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in vol_ents]
+    N = mesh.N
+    out += ["$EndEntities", "$Nodes", f"1 {N} 1 {N}", f"3 1 0 {N}"]
+    out += [str(i + 1) for i in range(N)] + [" ".join(f"{v:.17g}" for v in x) for x in mesh.coords]
+    ents = [(2, t, et, c) for t, _, et, c in surf_ents] + [(3, t, et, c) for t, _, et, c in vol_ents]
+    ne = sum(c.shape[0] for *_, c in ents)
+    out += ["$EndNodes", "$Elements", f"{len(ents)} {ne} 1 {ne}"]
+    tag = 1
+    for dim, t, et, c in ents:
+        out.append(f"{dim} {t} {et} {c.shape[0]}")
+        for row in c:
+            out.append(f"{tag} " + " ".join(str(v + 1) for v in row))
+            tag += 1
+    out.append("$EndElements")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")

@@ -69,3 +69,27 @@ def test_reader_errors(tmp_path):
                    "$EndNodes\n$Elements\n0 0 0 0\n$EndElements\n")
     with pytest.raises(ValueError, match="node tags must be 1..N"):
         read_msh(gap)
+
+
+def test_mixed_volume_types_round_trip(tmp_path):
+    """A file with hexahedra and wedges: one block per volume element type
+    (types in order of first appearance, elements in file order), material
+    groups per block, and a face group of two face types as a list."""
+    from mixed_mesh import hex_prism_box, write_msh
+    m = hex_prism_box()
+    path = tmp_path / "hexprism.msh"
+    write_msh(m, path)
+    r = read_msh(path)
+    blocks = r.element_blocks()
+    assert [b.etype for b in blocks] == [5, 6]
+    for got, want in zip(blocks, m.element_blocks()):
+        for g in ("steel", "alu"):
+            a = got.conn[got.volumes[g]]
+            b = want.conn[want.volumes[g]]
+            assert np.array_equal(a, b)
+        assert got.conn.shape == want.conn.shape
+    assert np.allclose(r.coords, m.coords, rtol=0, atol=1e-15)
+    front = r.face_sets("front")
+    assert [fs.etype for fs in front] == [3, 2]
+    assert np.array_equal(np.unique(r.nodes_on("front")), np.unique(m.nodes_on("front")))
+    assert np.array_equal(r.nodes_on("steel"), m.nodes_on("steel"))
