@@ -6,7 +6,7 @@
   nu = 0.3).  A step is one full assembly of K into the device-resident CSC
   values with the mesh, pattern and element->nnz map already in HBM.
 --workload hex125: FP64 stiffness of the 12^3 order-4 hexahedron cube (1,728
-  hex125, interior nodes perturbed by +-5% of h, 5x5x5 Gauss) on the matrix
+  hex125, interior nodes perturbed by +-2% of h, 5x5x5 Gauss) on the matrix
   cores, with the BF16x3 variant and its error against FP64 reported.
 --workload batch256: 256 seeded uneven problems (hex8 / tet10 / wedge6,
   2k-400k DoF), stiffness + consistent mass of all of them in one persistent
@@ -221,7 +221,7 @@ def bench_hex125(args):
     BF16x3 with FP32 accumulation)."""
     from lowlevelfem import _lib, mesh as M, refelem as R
     n = int(args.cells) if args.cells else 12
-    m = M.structured_box_mesh(n=n, order=4, perturb=0.05, seed=12)
+    m = M.structured_box_mesh(n=n, order=4, perturb=0.02, seed=12)
     tab = R.rule_tables(m.etype, 9)
     npe, q = m.npe, tab.q
     ctx = _lib.default_context()
@@ -263,7 +263,7 @@ def bench_hex125(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"hex125 unit cube {n}^3, interior nodes perturbed +-5% of h (seed 12), "
+        "config": {"workload": f"hex125 unit cube {n}^3, interior nodes perturbed +-2% of h (seed 12), "
                                "5x5x5 Gauss, FP64 stiffness into CSC",
                    "elements": m.E, "nodes": m.N, "dofs": 3 * m.N, "nnz": nnz},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -300,7 +300,7 @@ def cpu_baseline_hex125(n=6):
     rule and perturbation as the GPU workload), all threads."""
     from lowlevelfem import mesh as M, refelem as R
     from oracle import oracle as O
-    m = M.structured_box_mesh(n=n, order=4, perturb=0.05, seed=12)
+    m = M.structured_box_mesh(n=n, order=4, perturb=0.02, seed=12)
     tab = R.rule_tables(m.etype, 9)
     cp, rv = O.csc_pattern(m)
     T = O.num_threads()
