@@ -214,6 +214,16 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
   }
   LLFEM_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
+  // LLFEM_BATCH_TRACE=1: wall time of the host phases on stderr
+  const bool trace = std::getenv("LLFEM_BATCH_TRACE") != nullptr;
+  auto tmark = [&](const char* what) {
+    if (!trace) return;
+    static thread_local auto last = std::chrono::steady_clock::now();
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[batch] %s %.3f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+    last = now;
+  };
+  tmark("enter");
 
   // ---- classify; rule tables and affine checks ---------------------------------
   std::vector<Item> items(n);
@@ -287,6 +297,7 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     else LLFEM_TRY(cached_table(ctx, tkey, operator_table(it.d, it.m->npe, it.affine), &it.tab));
   }
 
+  tmark("classify + tables");
   // ---- geometry segments and panel segments ------------------------------------
   std::vector<GeoSeg> gsegs;
   std::vector<BatchSeg> bsegs;
@@ -394,6 +405,7 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
   }
   if (items_total >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "llfem_assemble_batch: too many work items");
 
+  tmark("segments");
   float ms_all = 0.f, ms_main = 0.f;
   if (!bsegs.empty()) {
     const size_t gbytes = sizeof(GeoSeg) * gsegs.size(), bbytes = sizeof(BatchSeg) * bsegs.size();
@@ -434,7 +446,9 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
       }
     }
     LLFEM_HIP(hipEventRecord(ctx->ev[3], st));
+    tmark("upload + launches");
     LLFEM_HIP(hipEventSynchronize(ctx->ev[3]));
+    tmark("wait for the device");
     LLFEM_HIP(hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[3]));
     LLFEM_HIP(hipEventElapsedTime(&ms_main, ctx->ev[1], ctx->ev[2]));
   }

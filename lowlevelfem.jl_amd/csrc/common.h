@@ -259,7 +259,12 @@ class BatchSolver {
   int update(SolveSys* s, bool init);
   int rebuild_tables();
   void release(SolveSys* s);
+  int launch_chunk(int iters);       // the iterations of one chunk, as kernel launches
+  void drop_graph();
   llfem_ctx* ctx_;
+  // the chunk captured as a HIP graph, valid while the live set is unchanged
+  hipGraphExec_t chunk_exec_ = nullptr;
+  int chunk_iters_ = 0;
   bool f64v_;
   std::vector<SolveSys*> sys_, live_;
   bool dirty_ = true;

@@ -528,6 +528,7 @@ struct SolveSys {
 BatchSolver::BatchSolver(llfem_ctx* c, bool f64_vectors) : ctx_(c), f64v_(f64_vectors) {}
 
 BatchSolver::~BatchSolver() {
+  drop_graph();
   for (SolveSys* s : sys_) release(s);
   sys_.clear();
   if (tables_) dfree(ctx_, tables_, tables_bytes_);
@@ -814,26 +815,49 @@ int BatchSolver::rebuild_tables() {
   return LLFEM_OK;
 }
 
+void BatchSolver::drop_graph() {
+  if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
+  chunk_exec_ = nullptr;
+  chunk_iters_ = 0;
+}
+
+// Whether chunks run as a captured graph: on unless LLFEM_ENGINE_GRAPH=0.
+static bool engine_graph_enabled() {
+  const char* e = std::getenv("LLFEM_ENGINE_GRAPH");
+  return !(e && e[0] == '0');
+}
+
 int BatchSolver::run(int iters, std::vector<SolveSys*>* finished) {
   hipStream_t st = ctx_->stream;
-  if (dirty_) LLFEM_TRY(rebuild_tables());
+  if (dirty_) {
+    drop_graph();
+    LLFEM_TRY(rebuild_tables());
+  }
   if (nsys_ == 0) return LLFEM_OK;
   const SysDev* sys = static_cast<const SysDev*>(d_sys_);
-  for (int k = 0; k < iters; ++k) {
-    if (f64v_) {
-      k_bs_spmv<double><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
-      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
-      k_bs_update<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
-      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
-      k_bs_pupd<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
-    } else {
-      k_bs_spmv<float><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
-      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
-      k_bs_update<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
-      LLFEM_TRY(amg_batch_apply(ctx_, amgb_));
-      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
-      k_bs_pupd<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+  if (engine_graph_enabled()) {
+    // The chunk's launches (SpMV, scalars, updates, V-cycles) depend only on
+    // the live set's tables; the per-system state they read and write lives
+    // in device memory.  Capture them once per live set and replay.
+    if (!chunk_exec_ || chunk_iters_ != iters) {
+      drop_graph();
+      hipGraph_t g = nullptr;
+      LLFEM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = launch_chunk(iters);
+      const hipError_t ec = hipStreamEndCapture(st, &g);
+      LLFEM_TRY(rc);
+      if (ec != hipSuccess) return fail(LLFEM_EHIP, std::string("engine graph capture: ") + hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&chunk_exec_, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        chunk_exec_ = nullptr;
+        return fail(LLFEM_EHIP, std::string("engine graph instantiate: ") + hipGetErrorString(ei));
+      }
+      chunk_iters_ = iters;
     }
+    LLFEM_HIP(hipGraphLaunch(chunk_exec_, st));
+  } else {
+    LLFEM_TRY(launch_chunk(iters));
   }
   k_bs_states<<<blocks(nsys_), kSB, 0, st>>>(sys, nsys_, d_states_);
   std::vector<double> h(2 * (size_t)nsys_);
@@ -850,6 +874,28 @@ int BatchSolver::run(int iters, std::vector<SolveSys*>* finished) {
         dirty_ = true;
         if (finished) finished->push_back(s);
       }
+    }
+  }
+  return LLFEM_OK;
+}
+
+int BatchSolver::launch_chunk(int iters) {
+  hipStream_t st = ctx_->stream;
+  const SysDev* sys = static_cast<const SysDev*>(d_sys_);
+  for (int k = 0; k < iters; ++k) {
+    if (f64v_) {
+      k_bs_spmv<double><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
+      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_update<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_pupd<double><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+    } else {
+      k_bs_spmv<float><<<(unsigned)nsb_, kSB, 0, st>>>(sys, d_sb_);
+      k_bs_alpha<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_update<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
+      LLFEM_TRY(amg_batch_apply(ctx_, amgb_));
+      k_bs_beta<<<(unsigned)nsys_, kSB, 0, st>>>(sys);
+      k_bs_pupd<float><<<(unsigned)nvb_, kSB, 0, st>>>(sys, d_vb_);
     }
   }
   return LLFEM_OK;

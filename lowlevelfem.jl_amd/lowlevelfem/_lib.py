@@ -186,23 +186,34 @@ def make_desc(op, tables, E=2e5, nu=0.3, rho=7.85e-9, D=None, lumped=False, prec
     return desc, (w, Nr, dN, Dm, sc)
 
 
+class Batch:
+    """The argument arrays of llfem_assemble_batch for a list of
+    (DeviceMesh, op, tables, kwargs) items, built once; run() makes the call
+    (one persistent launch; values stay on the device, DeviceMesh.values)."""
+
+    def __init__(self, items):
+        self.ctx = items[0][0].ctx if items else None
+        self.n = len(items)
+        self.meshes = [it[0] for it in items]                  # keep the handles alive
+        self.handles = (C.c_void_p * self.n)(*[it[0].h.value for it in items])
+        self.descs = (OpDesc * self.n)()
+        self.keep = []
+        for i, it in enumerate(items):
+            d, k = make_desc(it[1], it[2], **(it[3] if len(it) > 3 else {}))
+            self.descs[i] = d
+            self.keep.append(k)
+
+    def run(self):
+        st = Stats()
+        if self.n:
+            check(self.ctx.lib.llfem_assemble_batch(self.n, self.handles, self.descs, C.byref(st)))
+        return st
+
+
 def assemble_batch(items):
     """One persistent assembly launch over many (DeviceMesh, op, tables, kwargs)
     items (llfem_assemble_batch); values stay on the device (DeviceMesh.values)."""
-    if not items:
-        return Stats()
-    ctx = items[0][0].ctx
-    n = len(items)
-    handles = (C.c_void_p * n)(*[it[0].h.value for it in items])
-    descs = (OpDesc * n)()
-    keep = []
-    for i, it in enumerate(items):
-        d, k = make_desc(it[1], it[2], **(it[3] if len(it) > 3 else {}))
-        descs[i] = d
-        keep.append(k)
-    st = Stats()
-    check(ctx.lib.llfem_assemble_batch(n, handles, descs, C.byref(st)))
-    return st
+    return Batch(items).run()
 
 
 class DeviceMesh:
