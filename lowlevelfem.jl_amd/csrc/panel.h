@@ -327,6 +327,46 @@ __device__ __forceinline__ void hooke_block(Real lam, Real mu, const Real* G, Re
   K[8] += tr;
 }
 
+// Isotropic Hooke on an affine element, accumulated into the panel rows:
+//   K0 = Minv^T T' Minv with T' = lam Gh + mu Gh^T   (= lam G + mu G^T),
+//   tr(K0) = (lam + mu) tr(G), so mu tr(G) = cmu tr(K0), cmu = mu / (lam + mu),
+//   dst[c][d] += wd (K0[d][c] + cmu tr(K0) delta_dc).
+// Tp is T' (prime_entry below builds it from Gh); the wd scale is folded into
+// the accumulating fma.  Written as explicit fused multiply-adds so every
+// kernel that shares it rounds the same way.
+template <typename Real>
+__device__ __forceinline__ Real prime_entry(Real lam, Real mu, const Real* Gh, int k) {
+  return __builtin_fma(lam, Gh[k], mu * Gh[(k % 3) * 3 + k / 3]);
+}
+
+template <typename Real>
+__device__ __forceinline__ void hooke_affine_acc(const Geo<Real>& g, const Real* Tp, Real cmu, const double* old,
+                                                 double* out) {
+  const Real* Mi = g.Mi;
+  Real M[9];   // M[al][k] = sum_be Tp[al][be] Mi[be][k]
+#pragma unroll
+  for (int al = 0; al < 3; ++al)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      M[al * 3 + kk] =
+          __builtin_fma(Tp[al * 3 + 2], Mi[6 + kk], __builtin_fma(Tp[al * 3 + 1], Mi[3 + kk], Tp[al * 3] * Mi[kk]));
+  Real K0[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      K0[i * 3 + kk] = __builtin_fma(Mi[6 + i], M[6 + kk], __builtin_fma(Mi[3 + i], M[3 + kk], Mi[i] * M[kk]));
+  const Real t = cmu * (K0[0] + K0[4] + K0[8]);
+  K0[0] += t;
+  K0[4] += t;
+  K0[8] += t;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      out[c * 3 + d] = __builtin_fma((double)g.wd, (double)K0[d * 3 + c], old[c * 3 + d]);
+}
+
 // K[d][c] = Ke[(b,d),(a,c)] from G and the material.
 template <bool ISO, typename Real>
 __device__ __forceinline__ void block_K(const PanelParams& P, const Real* G, Real scale, Real* K) {
@@ -372,11 +412,36 @@ __device__ __forceinline__ TabBlk<Real> load_tab(const double* tab, const float*
 
 // Add element block Ke[(b,:),(a,:)] to the panel rows of slot s (or, for a
 // non-affine element, integrate it point by point first).
-template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real>
+// PRIMED: the table entry tb already holds T' = lam Gh + mu Gh^T (the round
+// kernels build it once per block in LDS); otherwise it is formed here.
+template <int NPE, typename MapT, int OP, bool AFFINE, bool ISO, typename Real, bool PRIMED = false>
 __device__ __forceinline__ void contribute(const PanelParams& P, const double* geo, const double* tab,
                                            const TabBlk<Real>& tb, double* __restrict__ wl, int base, int R,
                                            int b, int e, int a, int s, const Geo<Real>& g, int mofs = 0) {
-  if constexpr (OP == LLFEM_OP_STIFFNESS || OP == kOpKM) {
+  if constexpr ((OP == LLFEM_OP_STIFFNESS || OP == kOpKM) && AFFINE && ISO) {
+    Real Tp[9];
+    if constexpr (PRIMED) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Tp[k] = tb.v[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Tp[k] = prime_entry<Real>((Real)P.lam, (Real)P.mu, tb.v, k);
+    }
+    const Real cmu = (Real)(P.mu / (P.lam + P.mu));
+    double* dst = wl + base + 3 * s;
+    double old[9], upd[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) old[c * 3 + d] = dst[c * 3 * R + d];
+    hooke_affine_acc<Real>(g, Tp, cmu, old, upd);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dst[c * 3 * R + d] = upd[c * 3 + d];
+    if constexpr (OP == kOpKM) lds_add(wl + base + mofs + s, P.rho * (double)g.wd * (double)tb.m);
+    __builtin_amdgcn_wave_barrier();
+  } else if constexpr (OP == LLFEM_OP_STIFFNESS || OP == kOpKM) {
     Real G[9];
     Real scale;
     if constexpr (AFFINE) {
@@ -754,7 +819,7 @@ __device__ __forceinline__ void panel_wave_rounds(const PanelParams& P, int64_t 
       const RawRec& cr = rec[u % (2 * PFX)];
       const int ca = raw_loc(cr);
       const bool live = node_live && ca >= 0;
-      contribute<NPE, uint8_t, OP, true, true, Real>(P, geo, tab, tb[u % PFX], live ? pan : dummy, 0,
+      contribute<NPE, uint8_t, OP, true, true, Real, TL>(P, geo, tab, tb[u % PFX], live ? pan : dummy, 0,
                                                      live ? dcur.y : 0, b, raw_elem(cr), ca < 0 ? 0 : ca,
                                                      live ? raw_slot(cr, b) : 0, g[u % PFX], 0);
       // ring refill: geometry/table for step s+PFX, record for step s+2PFX
@@ -810,15 +875,20 @@ __global__ __launch_bounds__(512) void k_panel_rounds(PanelParams P) {
     tl = reinterpret_cast<Real*>(lds + (((size_t)(blockDim.x >> 6) * P.lds_per_wave + 1) & ~(size_t)1));
     const double* tab = (const double*)gp(P.tab);
     const float* tabf = (const float*)gp(P.tabf);
+    // T' = lam Gh + mu Gh^T per (a, b) entry (isotropic stiffness:
+    // contribute<..., PRIMED>), one entry per thread
     constexpr int TS = kTabLds<Real>;
-    for (int i = threadIdx.x; i < NPE * NPE * TS; i += blockDim.x) {
-      const int ent = i / TS, k = i - ent * TS;
-      Real v = 0;
-      if (k < 9) {
-        if constexpr (sizeof(Real) == 8) v = tab[ent * 9 + k];
-        else v = tabf[ent * 9 + k];
+    for (int ent = threadIdx.x; ent < NPE * NPE; ent += blockDim.x) {
+      Real gh[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+        if constexpr (sizeof(Real) == 8) gh[m] = tab[ent * 9 + m];
+        else gh[m] = tabf[ent * 9 + m];
       }
-      tl[i] = v;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) tl[ent * TS + k] = prime_entry<Real>((Real)P.lam, (Real)P.mu, gh, k);
+#pragma unroll
+      for (int k = 9; k < TS; ++k) tl[ent * TS + k] = 0;
     }
     __syncthreads();
   }
