@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "common.h"
+#include "engine_policy.h"
 
 namespace llfem {
 
@@ -512,15 +513,14 @@ struct SolveSys {
   void* mem = nullptr;
   size_t bytes = 0;
   double *b = nullptr, *xh = nullptr, *xt = nullptr, *rh = nullptr, *y = nullptr, *scal = nullptr, *part = nullptr;
-  int bad = 0;              // consecutive updates whose FP32 correction was not finite
-  int64_t rollbacks = 0;    // updates that kept the previous solution (non-finite correction)
+  UpdateBook book;          // rollbacks, consecutive non-finite corrections, updates
   int64_t refreshes = 0;    // AMG updates whose z had to be recomputed (drift or r.z <= 0)
   void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;   // Krylov vectors (VT)
   float *binv = nullptr, *val32 = nullptr;
   int64_t npart = 0;
   double bnorm = 0.0, scale = 1.0, relres = 0.0;
-  int64_t iters = 0, updates = 0;
-  int status = 0;           // 0 running, 1 converged, -1 maxiter, -2 non-finite FP32 corrections
+  int64_t iters = 0;
+  int status = kRunning;
   int32_t sblk0 = 0, nsblk = 0, vblk0 = 0, nvblk = 0;
   AmgSys* amg = nullptr;
 };
@@ -579,14 +579,11 @@ int BatchSolver::update(SolveSys* s, bool init) {
     // finite (a host-side switch for testing the rollback path)
     const char* force_env = std::getenv("LLFEM_ENGINE_FORCE_ROLLBACK");
     const int64_t force_rb = force_env ? (int64_t)std::atoll(force_env) : 0;
-    if (force_rb > 0 && s->updates + 1 == force_rb && s->rollbacks == 0) r2 = NAN;
-    if (std::isfinite(r2)) {
+    if (force_rb > 0 && s->book.updates + 1 == force_rb && s->book.rollbacks == 0) r2 = NAN;
+    if (adopt_trial(r2, &s->book)) {
       std::swap(s->xh, s->xt);
-      s->bad = 0;
     } else {
-      // the FP32 correction overflowed: keep xh, restart the recursion from it
-      ++s->bad;
-      ++s->rollbacks;
+      // rollback: keep xh, restart the recursion from it
       fresh = true;
       LLFEM_TRY(residual(s->xh, &r2));
     }
@@ -598,25 +595,11 @@ int BatchSolver::update(SolveSys* s, bool init) {
   const double rn = std::sqrt(r2);
   s->iters = (int64_t)it;
   if (init) s->bnorm = rn;
-  else ++s->updates;
-  if (s->bad >= 3 || !std::isfinite(rn)) {
-    s->status = -2;
-    s->relres = s->bnorm > 0 && std::isfinite(rn) ? rn / s->bnorm : 0.0;
-    const double idle = ST_IDLE;
-    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
-    LLFEM_HIP(hipStreamSynchronize(st));
-    return LLFEM_OK;
-  }
-  if (rn <= sp.reltol * s->bnorm || s->bnorm == 0.0) {
-    s->status = 1;
-    const double idle = ST_IDLE;
-    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
-    LLFEM_HIP(hipStreamSynchronize(st));
-    return LLFEM_OK;
-  }
-  if (s->iters >= sp.maxiter || s->updates > 100000) {
-    s->status = -1;
-    s->relres = s->bnorm > 0 ? rn / s->bnorm : 0.0;
+  else ++s->book.updates;
+  const int status = update_status(rn, s->bnorm, sp.reltol, s->iters, sp.maxiter, s->book);
+  if (status != kRunning) {
+    s->status = status;
+    if (status != kConverged) s->relres = s->bnorm > 0 && std::isfinite(rn) ? rn / s->bnorm : 0.0;
     const double idle = ST_IDLE;
     LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
     LLFEM_HIP(hipStreamSynchronize(st));
@@ -649,7 +632,7 @@ int BatchSolver::update(SolveSys* s, bool init) {
       LLFEM_HIP(hipMemcpyAsync(&h[1], s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
       LLFEM_HIP(hipStreamSynchronize(st));
       // r is the true residual scaled to unit norm, so rr = 1 up to rounding
-      if (!(h[0] > 0.0) || !(h[1] <= kDriftAmg * kDriftAmg)) {
+      if (!amg_keep_rescaled(h[0], h[1], kDriftAmg)) {
         ++s->refreshes;
         k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, 1, (float*)s->r,
                                                           (float*)s->p, vpart);
@@ -937,8 +920,8 @@ bool BatchSolver::any_running() const {
 }
 
 int64_t sys_iterations(const SolveSys* s) { return s->iters; }
-int64_t sys_updates(const SolveSys* s) { return s->updates; }
-int64_t sys_rollbacks(const SolveSys* s) { return s->rollbacks; }
+int64_t sys_updates(const SolveSys* s) { return s->book.updates; }
+int64_t sys_rollbacks(const SolveSys* s) { return s->book.rollbacks; }
 int64_t sys_refreshes(const SolveSys* s) { return s->refreshes; }
 int sys_status(const SolveSys* s) { return s->status; }
 double sys_relres(const SolveSys* s) { return s->relres; }
