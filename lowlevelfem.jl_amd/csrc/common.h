@@ -161,19 +161,23 @@ struct llfem_mesh {
   float* d_valK32 = nullptr;
   double* d_diagK = nullptr;
   bool solver_ready = false;
-  // staged high-order assembly: per nnz block, its element blocks in
-  // ascending element order (d_csrc[d_cptr[p] .. d_cptr[p+1]))
-  int64_t ncontrib = 0;
-  int64_t* d_cptr = nullptr;      // [nadj+1]
-  int32_t* d_csrc = nullptr;      // [ncontrib] staged block index e*npe^2 + a*npe + b
-  int32_t* d_blk_node = nullptr;  // [nadj] column node of each nnz block
-  int32_t* d_direct = nullptr;    // [E*npe^2] CSC block a staged block is the only contribution to, else -1
-  int32_t* d_multi = nullptr;     // [nmulti] CSC blocks with more than one contribution
-  int64_t nmulti = 0;
+  // high-order (Gram) assembly tables, built once per mesh.  Positions are
+  // the Gram row order: local node d_ho_perm[pos] (element 0's nodes in
+  // ascending global id).  An element block (column pos, row pos) that is the
+  // only contribution to its CSC block is written straight into the column's
+  // panel; the others are staged in the order of their CSC block and, within
+  // it, ascending element, and summed by the gather.
+  std::vector<int32_t> h_conn0;   // nodes of element 0 (0-based)
+  int32_t* d_ho_perm = nullptr;   // [npe] local node of each row position
+  int32_t* d_gslot = nullptr;     // [E*npe*npe] (e, col pos, row pos): >= 0: offset 3 s of the block in its
+                                  // column panel (slot s); < 0: -(k+1) for staged block k
+  int64_t* d_gcol = nullptr;      // [E*npe][2] (e, col pos): 9 a0 and 3 R of the column node's panel
+  int64_t nstaged = 0;            // staged element blocks
+  int64_t nmulti = 0;             // CSC blocks with more than one contribution
+  int32_t* d_mptr = nullptr;      // [nmulti+1] their staged blocks d_mptr[m] .. d_mptr[m+1]
+  int64_t* d_mref = nullptr;      // [nmulti][2] value offset 9 a0 + 3 s and panel stride 3 R of each
   void* d_ho = nullptr;           // staged blocks + Z
   size_t ho_bytes = 0;
-  std::vector<int32_t> h_conn0;   // nodes of element 0 (0-based)
-  int32_t* d_ho_perm = nullptr;   // [npe] local node of each Gram row position (ascending node id)
   // AMG hierarchies kept per support (key: hash of the constrained-DoF list),
   // each with the device mask it was built for
   std::map<uint64_t, std::pair<llfem::AmgHier*, uint8_t*>> amg;

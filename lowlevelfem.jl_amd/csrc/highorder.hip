@@ -11,8 +11,11 @@
 //   FP64    v_mfma_f64_16x16x4f64,
 //   BF16x3  v_mfma_f32_16x16x16bf16_1k with Z split into bf16 hi + lo parts,
 //           G ~ Zh Zh^T + Zh Zl^T + Zl Zh^T, FP32 accumulation.
-// Element blocks are staged in FP64 and summed into the CSC by a
-// deterministic segmented gather (contributions in ascending element order).
+// An element block that is the only contribution to its CSC block is written
+// by the Gram epilogue straight into the column's panel; the others are
+// staged in FP64, contiguous per CSC block in ascending element order, and
+// summed by a gather.  The epilogue's destinations come from per-mesh tables
+// indexed by the Gram row positions (no dependent index loads per block).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -94,6 +97,8 @@ __global__ void k_hex_z(int64_t E, int npe, int q, int rows, int affine, const d
   }
 }
 
+constexpr int kGramThreads = 256;   // 4 waves; the epilogue deals 2 x 256 blocks per pass
+
 struct GramParams {
   int64_t E;
   int npe;
@@ -104,15 +109,10 @@ struct GramParams {
   const short* Zh;
   const short* Zl;
   double lam, mu;
-  double C[81];
-  int iso;
-  double* S;              // staged blocks S[e][col node a][row node b][c][d]
-  // a block that is the only contribution to its CSC block goes straight to
-  // the CSC values (direct[staged index] = CSC block, -1: staged)
-  const int32_t* direct;
-  const int32_t* perm;    // [npe] local node of each row position
-  const int32_t* blk_node;
-  const int64_t* adj_ptr;
+  const double* C;        // general material [d][x][c][y] in device memory (NULL: isotropic lam, mu)
+  double* S;              // staged blocks [k][c][d]
+  const int32_t* gslot;   // [E][col pos][row pos]: >= 0 offset in the column panel, < 0: -(k+1) staged
+  const int64_t* gcol;    // [E][col pos][2]: 9 a0 and 3 R of the column node's panel
   double* out;
 };
 
@@ -136,68 +136,93 @@ __device__ __forceinline__ bool gram_block(const GramParams& P, int64_t& e, int&
   return true;
 }
 
-// Epilogue: LDS holds rows [48h, 48h+48) of the 96x96 G tile (row nodes
-// 16h..16h+15 of tile ti, all 32 column nodes of tile tj) -> staged K blocks.
-__device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, int ti, int tj, int h) {
+// Column references of a tile pair into LDS (read after the first barrier):
+// slots 0..31 the positions of tile tj, 32..63 those of tile ti.
+__device__ __forceinline__ void load_colref(const GramParams& P, int64_t e, int ti, int tj, int64_t* colref) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int slot = t >> 1;
+    const int pos = slot < 32 ? tj * 32 + slot : ti * 32 + slot - 32;
+    colref[t] = pos < P.npe ? P.gcol[2 * ((size_t)e * P.npe + pos) + (t & 1)] : 0;
+  }
+}
+
+// Epilogue: LDS holds rows [48h, 48h+48) of the 96x96 G tile (row positions
+// 16h..16h+15 of tile ti, all 32 column positions of tile tj) -> K blocks,
+// written in place (single contribution) or staged.  Each thread takes two
+// blocks per pass; the destination references of all its blocks are loaded
+// before any block is formed, so their latencies overlap.
+template <bool ISO>
+__device__ void gram_epilogue(const GramParams& P, const double* Gt, const int64_t* colref, const double* Cs,
+                              int64_t e, int ti, int tj, int h) {
   const int npe = P.npe;
-  double* S = P.S + (size_t)e * npe * npe * 9;
-  for (int pass = 0; pass < (ti == tj ? 1 : 2); ++pass) {
-    for (int idx = threadIdx.x; idx < 16 * 32; idx += blockDim.x) {
-      // pass 0: block (row b in ti, col a in tj), b fastest across threads;
-      // pass 1: its transpose (row a in tj, col b in ti), a fastest.
-      const int hl = pass == 0 ? (idx & 15) : (idx >> 5);   // row node within the half
-      const int al = pass == 0 ? (idx >> 4) : (idx & 31);   // col node in tile tj
-      const int b = ti * 32 + 16 * h + hl, a = tj * 32 + al;
-      if (b >= npe || a >= npe) continue;
-      // G_ba[x][y] of (row node b, col node a); pass 1 needs G_ab = G_ba^T
-      double G[9];
+  const int npass = ti == tj ? 1 : 2;
+  int32_t sl[4];
+  bool ok[4];
 #pragma unroll
-      for (int x = 0; x < 3; ++x)
+  for (int k = 0; k < 4; ++k) {
+    // pass 0: block (row b in ti, col a in tj), b fastest across threads;
+    // pass 1: its transpose (row a in tj, col b in ti), a fastest.
+    const int pass = k >> 1, idx = threadIdx.x + kGramThreads * (k & 1);
+    const int hl = pass == 0 ? (idx & 15) : (idx >> 5);   // row position within the half
+    const int al = pass == 0 ? (idx >> 4) : (idx & 31);   // column position in tile tj
+    const int b = ti * 32 + 16 * h + hl, a = tj * 32 + al;
+    ok[k] = pass < npass && b < npe && a < npe;
+    sl[k] = ok[k] ? P.gslot[((size_t)e * npe + (pass == 0 ? a : b)) * npe + (pass == 0 ? b : a)] : 0;
+  }
+  // one block at a time (the other half's accumulators are still live)
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    const int32_t slk = k == 0 ? sl[0] : k == 1 ? sl[1] : k == 2 ? sl[2] : sl[3];
+    const bool okk = k == 0 ? ok[0] : k == 1 ? ok[1] : k == 2 ? ok[2] : ok[3];
+    if (!okk) continue;
+    const int pass = k >> 1, idx = threadIdx.x + kGramThreads * (k & 1);
+    const int hl = pass == 0 ? (idx & 15) : (idx >> 5);
+    const int al = pass == 0 ? (idx >> 4) : (idx & 31);
+    // G_ba[x][y] of (row position b, column position a); pass 1 needs G_ab = G_ba^T
+    double G[9];
 #pragma unroll
-        for (int y = 0; y < 3; ++y) {
-          const double g = Gt[(3 * hl + x) * kHoTile + 3 * al + y];
-          if (pass == 0) G[x * 3 + y] = g; else G[y * 3 + x] = g;
-        }
-      double K[9];
-      if (P.iso) {
-        const double tr = P.mu * (G[0] + G[4] + G[8]);
+    for (int x = 0; x < 3; ++x)
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) K[d * 3 + c] = P.lam * G[d * 3 + c] + P.mu * G[c * 3 + d];
-        K[0] += tr; K[4] += tr; K[8] += tr;
-      } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-#pragma unroll
-              for (int y = 0; y < 3; ++y) acc += P.C[((d * 3 + x) * 3 + c) * 3 + y] * G[x * 3 + y];
-            K[d * 3 + c] = acc;
-          }
+      for (int y = 0; y < 3; ++y) {
+        const double g = Gt[(3 * hl + x) * kHoTile + 3 * al + y];
+        if (pass == 0) G[x * 3 + y] = g; else G[y * 3 + x] = g;
       }
-      // block of column node cn, row node rn (local nodes): [c][d] = K[d][c]
-      const int bl = P.perm[b], al_ = P.perm[a];
-      const int cn = pass == 0 ? al_ : bl, rn = pass == 0 ? bl : al_;
-      const int32_t p = P.direct[((size_t)e * npe + cn) * npe + rn];
-      if (p >= 0) {   // CSC panel of node j: value (c, slot s, d) at 9 a0 + c 3R + 3 s + d
-        const int64_t j = P.blk_node[p];
-        const int64_t a0 = P.adj_ptr[j];
-        const int64_t R = P.adj_ptr[j + 1] - a0;
-        double* o = P.out + 9 * a0 + 3 * (p - a0);
+    // destination of the block of (column node, row node): [c][d] at o[c * len + d]
+    double* o;
+    int64_t len;
+    if (slk >= 0) {   // CSC panel: value (c, slot s, d) at 9 a0 + c 3R + 3 s + d
+      // column reference in LDS: tile tj's 32 positions, then tile ti's
+      const int cslot = pass == 0 ? al : 32 + 16 * h + hl;
+      o = P.out + colref[2 * cslot] + slk;
+      len = colref[2 * cslot + 1];
+    } else {
+      o = P.S + (size_t)(-(int64_t)slk - 1) * 9;
+      len = 3;
+    }
+    if constexpr (ISO) {
+      double K[9];
+      const double tr = P.mu * (G[0] + G[4] + G[8]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) o[c * 3 * R + d] = K[d * 3 + c];
-      } else {
-        double* o = S + ((size_t)cn * npe + rn) * 9;
+        for (int c = 0; c < 3; ++c) K[d * 3 + c] = P.lam * G[d * 3 + c] + P.mu * G[c * 3 + d];
+      K[0] += tr; K[4] += tr; K[8] += tr;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) o[c * 3 + d] = K[d * 3 + c];
+        for (int d = 0; d < 3; ++d) o[c * len + d] = K[d * 3 + c];
+    } else {
+      // K[d][c] = sum_xy C[d][x][c][y] G[x][y], one entry at a time
+#pragma unroll 1
+      for (int dc = 0; dc < 9; ++dc) {
+        const int d = dc / 3, c = dc - 3 * d;
+        double acc = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = 0; y < 3; ++y) acc += Cs[((d * 3 + x) * 3 + c) * 3 + y] * G[x * 3 + y];
+        o[c * len + d] = acc;
       }
     }
   }
@@ -207,12 +232,18 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, int64_t e, 
 // subtiles of the 96x96 tile (wave w: rows 48*(w/2).., cols 48*(w%2)..).
 // Fragments of step kb+1 are loaded while the MFMAs of step kb run.  The tile
 // goes through LDS in two 48-row halves (36 KB), so four blocks fit a CU.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+template <bool ISO>
+__global__ __launch_bounds__(kGramThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_gram_f64(GramParams P) {
   extern __shared__ double Gt[];   // 48 x 96
+  __shared__ int64_t colref[2 * 64];
+  __shared__ double Cs[ISO ? 1 : 81];
+  if constexpr (!ISO)
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) Cs[i] = P.C[i];
   int64_t e;
   int ti, tj;
   if (!gram_block(P, e, ti, tj)) return;   // block-uniform
+  load_colref(P, e, ti, tj, colref);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rows = P.ntiles * kHoTile;
   const int lr = lane & 15, lk = lane >> 4;
@@ -257,17 +288,23 @@ void k_gram_f64(GramParams P) {
             Gt[(16 * i + lk + 4 * r) * kHoTile + tc0 + 16 * j + lr] = acc[i][j][r];
     }
     __syncthreads();
-    gram_epilogue(P, Gt, e, ti, tj, h);
+    gram_epilogue<ISO>(P, Gt, colref, Cs, e, ti, tj, h);
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+template <bool ISO>
+__global__ __launch_bounds__(kGramThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_gram_bf16x3(GramParams P) {
   extern __shared__ double Gt[];   // 48 x 96
+  __shared__ int64_t colref[2 * 64];
+  __shared__ double Cs[ISO ? 1 : 81];
+  if constexpr (!ISO)
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) Cs[i] = P.C[i];
   int64_t e;
   int ti, tj;
   if (!gram_block(P, e, ti, tj)) return;   // block-uniform
+  load_colref(P, e, ti, tj, colref);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rows = P.ntiles * kHoTile;
   const int lr = lane & 15, lk = lane >> 4;
@@ -326,42 +363,34 @@ void k_gram_bf16x3(GramParams P) {
             Gt[(16 * i + 4 * lk + r) * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
     }
     __syncthreads();
-    gram_epilogue(P, Gt, e, ti, tj, h);
+    gram_epilogue<ISO>(P, Gt, colref, Cs, e, ti, tj, h);
     __syncthreads();
   }
 }
 
-// One thread per node-pair block p: sum its staged element blocks in ascending
-// element order and write the 3x3 block into the node panel.
-// CSC blocks with more than one contribution: sum the staged blocks in
-// ascending element order (the single-contribution ones were written by the
-// Gram epilogue).
-__global__ void k_stage_gather(int64_t nmulti, const int32_t* __restrict__ multi, const int32_t* __restrict__ blk_node,
-                               const int64_t* __restrict__ adj_ptr, const int64_t* __restrict__ cptr,
-                               const int32_t* __restrict__ csrc, const double* __restrict__ S,
-                               double* __restrict__ out) {
+// CSC blocks with more than one contribution: thread per block, its staged
+// blocks are contiguous (ascending element order) and summed in that order.
+__global__ void k_stage_gather(int64_t nmulti, const int32_t* __restrict__ mptr, const int64_t* __restrict__ mref,
+                               const double* __restrict__ S, double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nmulti) return;
-  const int64_t p = multi[k];
-  const int64_t j = blk_node[p];
-  const int64_t a0 = adj_ptr[j];
-  const int64_t R = adj_ptr[j + 1] - a0;
-  const int64_t s = p - a0;
+  const int64_t c0 = mptr[k], c1 = mptr[k + 1];
+  const int64_t base = mref[2 * k], len = mref[2 * k + 1];
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t c = cptr[p]; c < cptr[p + 1]; ++c) {
-    const double* blk = S + (size_t)csrc[c] * 9;
+  for (int64_t c = c0; c < c1; ++c) {
+    const double* blk = S + (size_t)c * 9;
 #pragma unroll
     for (int m = 0; m < 9; ++m) acc[m] += blk[m];
   }
-  double* o = out + 9 * a0 + 3 * s;
+  double* o = out + base;
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) o[c * 3 * R + d] = acc[c * 3 + d];
+    for (int d = 0; d < 3; ++d) o[c * len + d] = acc[c * 3 + d];
 }
 
-// contribution keys: (block p << 32) | staged-block index e*npe^2 + a*npe + b,
-// generated per (incidence, element node b)
+// contribution keys: (block p << 32) | element block e*npe^2 + a*npe + b
+// (column local node a, row local node b), generated per (incidence, b)
 __global__ void k_contrib_keys(int64_t ninc, int npe, const int32_t* __restrict__ inc_ptr_node,
                                const int32_t* __restrict__ inc_elem, const uint8_t* __restrict__ inc_loc,
                                const uint8_t* __restrict__ map8, const uint16_t* __restrict__ map16,
@@ -391,13 +420,81 @@ __global__ void k_blk_node(int64_t N, const int64_t* __restrict__ adj_ptr, int32
   for (int64_t p = adj_ptr[j]; p < adj_ptr[j + 1]; ++p) bn[p] = (int32_t)j;
 }
 
-__global__ void k_direct_map(int64_t nblk, const int64_t* __restrict__ cptr, const int32_t* __restrict__ csrc,
-                             int32_t* __restrict__ direct, uint8_t* __restrict__ multi_flag) {
+// per sorted contribution t: 1 if its CSC block has several contributions
+// (slot n: 0, so the exclusive sum ends with the staged count)
+__global__ void k_multi_flags(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ flag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n) return;
+  int32_t f = 0;
+  if (t < n) {
+    const uint64_t p = keys[t] >> 32;
+    f = (t > 0 && (keys[t - 1] >> 32) == p) || (t + 1 < n && (keys[t + 1] >> 32) == p);
+  }
+  flag[t] = f;
+}
+
+// gslot per sorted contribution t: staged blocks get -(sid+1); a single
+// contribution gets its value offset 3 s in the column panel.  Multi blocks
+// record their first staged block and their value offset (at the first of
+// their contributions).
+__global__ void k_gslot(int64_t n, int npe, const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
+                        const int32_t* __restrict__ sid, const int32_t* __restrict__ iperm,
+                        const int32_t* __restrict__ blk_node, const int64_t* __restrict__ adj_ptr,
+                        int32_t* __restrict__ gslot, int32_t* __restrict__ mfirst, int64_t* __restrict__ mref_of_blk) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t p = (int64_t)(keys[t] >> 32);
+  const int64_t src = (int64_t)(uint32_t)keys[t];
+  const int64_t nn = (int64_t)npe * npe;
+  const int64_t e = src / nn;
+  const int a = (int)((src / npe) % npe), b = (int)(src % npe);
+  const int64_t pos = (e * npe + iperm[a]) * npe + iperm[b];
+  const int64_t j = blk_node[p];
+  const int64_t a0 = adj_ptr[j];
+  if (flag[t]) {
+    gslot[pos] = -(sid[t] + 1);
+    if (t == 0 || (keys[t - 1] >> 32) != (uint64_t)p) {
+      mfirst[p] = sid[t];
+      mref_of_blk[2 * p] = 9 * a0 + 3 * (p - a0);
+      mref_of_blk[2 * p + 1] = 3 * (adj_ptr[j + 1] - a0);
+    }
+  } else {
+    gslot[pos] = (int32_t)(3 * (p - a0));
+  }
+}
+
+// per multi block m (ascending p): its staged range start and value reference
+__global__ void k_multi_tables(int64_t nmulti, const int32_t* __restrict__ multi, const int32_t* __restrict__ mfirst,
+                               const int64_t* __restrict__ mref_of_blk, int64_t nstaged, int32_t* __restrict__ mptr,
+                               int64_t* __restrict__ mref) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m > nmulti) return;
+  if (m == nmulti) {
+    mptr[m] = (int32_t)nstaged;
+    return;
+  }
+  const int64_t p = multi[m];
+  mptr[m] = mfirst[p];
+  mref[2 * m] = mref_of_blk[2 * p];
+  mref[2 * m + 1] = mref_of_blk[2 * p + 1];
+}
+
+// gcol per (element, column position): the column node's panel reference
+__global__ void k_gcol(int64_t E, int npe, const int32_t* __restrict__ conn, const int32_t* __restrict__ perm,
+                       const int64_t* __restrict__ adj_ptr, int64_t* __restrict__ gcol) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E * npe) return;
+  const int64_t e = i / npe;
+  const int apos = (int)(i - e * npe);
+  const int64_t j = conn[(size_t)perm[apos] * E + e];
+  const int64_t a0 = adj_ptr[j];
+  gcol[2 * i] = 9 * a0;
+  gcol[2 * i + 1] = 3 * (adj_ptr[j + 1] - a0);
+}
+
+__global__ void k_mark_multi(int64_t nadj, const int32_t* __restrict__ mfirst, uint8_t* __restrict__ flag) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nblk) return;
-  const int64_t n = cptr[p + 1] - cptr[p];
-  if (n == 1) direct[csrc[cptr[p]]] = (int32_t)p;
-  multi_flag[p] = n > 1 ? 1 : 0;
+  if (p < nadj) flag[p] = mfirst[p] >= 0 ? 1 : 0;
 }
 
 __global__ void k_fill_i32(int64_t n, int32_t v, int32_t* __restrict__ out) {
@@ -405,77 +502,119 @@ __global__ void k_fill_i32(int64_t n, int32_t v, int32_t* __restrict__ out) {
   if (i < n) out[i] = v;
 }
 
-__global__ void k_split_contrib(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ src,
-                                int64_t* __restrict__ ptr, int64_t nblk) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > n) return;
-  if (t < n) src[t] = (int32_t)(uint32_t)keys[t];
-  const int64_t cur = t < n ? (int64_t)(keys[t] >> 32) : nblk;
-  const int64_t prev = t > 0 ? (int64_t)(keys[t - 1] >> 32) : -1;
-  for (int64_t v = prev + 1; v <= cur; ++v) ptr[v] = t;
-}
-
 }  // namespace
 
-// Contribution lists for the staged gather, built once per mesh.
+// Gram tables of a mesh (d_ho_perm, d_gslot, d_gcol, d_mptr, d_mref), built
+// once: the element blocks' contributions sorted by (CSC block, element
+// block), then split into single-contribution blocks (written in place by
+// the Gram epilogue) and staged ones (contiguous per CSC block).
 int build_contrib(llfem_mesh* m) {
-  if (m->d_cptr) return LLFEM_OK;
+  if (m->d_gslot) return LLFEM_OK;
   llfem_ctx* ctx = m->ctx;
   hipStream_t st = ctx->stream;
-  const int64_t n = m->ninc * m->npe;
-  if (m->E * (int64_t)m->npe * m->npe >= ((int64_t)1 << 31))
+  const int npe = m->npe;
+  const int64_t n = m->ninc * npe;
+  if (m->E * (int64_t)npe * npe >= ((int64_t)1 << 31))
     return fail(LLFEM_EUNSUP, "staged assembly: E*npe^2 >= 2^31");
-  int32_t* node = nullptr;
+  if (n != m->E * (int64_t)npe * npe) return fail(LLFEM_EINVAL, "staged assembly: incidence count mismatch");
+  // row positions: element 0's nodes in ascending global id
+  std::vector<int32_t> perm(npe), iperm(npe);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return m->h_conn0[x] < m->h_conn0[y]; });
+  for (int k = 0; k < npe; ++k) iperm[perm[k]] = k;
+  int32_t *d_iperm = nullptr, *node = nullptr, *flag = nullptr, *sid = nullptr, *mfirst = nullptr,
+          *blk_node = nullptr, *multi = nullptr;
   uint64_t *keys = nullptr, *tmp = nullptr;
-  LLFEM_TRY(dalloc(ctx, &node, (size_t)m->ninc));
-  LLFEM_TRY(dalloc(ctx, &keys, (size_t)n));
-  LLFEM_TRY(dalloc(ctx, &tmp, (size_t)n));
-  k_node_of_incidence<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_inc_ptr, node);
-  k_contrib_keys<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
-      m->ninc, m->npe, node, m->d_inc_elem, m->d_inc_loc,
-      m->map_bytes == 1 ? (const uint8_t*)m->d_inc_map : nullptr,
-      m->map_bytes == 2 ? (const uint16_t*)m->d_inc_map : nullptr, m->d_adj_ptr, keys);
-  size_t tb = 0;
-  int end_bit = 64;
-  LLFEM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys, tmp, (int)n, 0, end_bit, st));
-  void* d_tmp = nullptr;
-  LLFEM_TRY(dmalloc(ctx, &d_tmp, tb));
-  LLFEM_HIP(hipcub::DeviceRadixSort::SortKeys(d_tmp, tb, keys, tmp, (int)n, 0, end_bit, st));
-  dfree(ctx, d_tmp, tb);
-  LLFEM_TRY(dalloc(ctx, &m->d_csrc, (size_t)n));
-  LLFEM_TRY(dalloc(ctx, &m->d_cptr, (size_t)m->nadj + 1));
-  LLFEM_TRY(dalloc(ctx, &m->d_blk_node, (size_t)m->nadj));
-  k_split_contrib<<<(unsigned)((n + 256) / 256), 256, 0, st>>>(n, tmp, m->d_csrc, m->d_cptr, m->nadj);
-  k_blk_node<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, m->d_blk_node);
-  // direct-write map of single-contribution blocks, list of the others
-  const int64_t nstaged = m->E * (int64_t)m->npe * m->npe;
-  uint8_t* flags = nullptr;
-  int64_t* d_num = nullptr;
-  LLFEM_TRY(dalloc(ctx, &m->d_direct, (size_t)nstaged));
-  LLFEM_TRY(dalloc(ctx, &flags, (size_t)m->nadj));
-  LLFEM_TRY(dalloc(ctx, &m->d_multi, (size_t)std::max<int64_t>(m->nadj, 1)));
-  LLFEM_TRY(dalloc(ctx, &d_num, 1));
-  k_fill_i32<<<(unsigned)((nstaged + 255) / 256), 256, 0, st>>>(nstaged, -1, m->d_direct);
-  k_direct_map<<<(unsigned)((m->nadj + 255) / 256), 256, 0, st>>>(m->nadj, m->d_cptr, m->d_csrc, m->d_direct, flags);
-  hipcub::CountingInputIterator<int32_t> ids(0);
-  size_t sb = 0;
-  LLFEM_HIP(hipcub::DeviceSelect::Flagged(nullptr, sb, ids, flags, m->d_multi, d_num, (int)m->nadj, st));
-  void* d_sel = nullptr;
-  LLFEM_TRY(dmalloc(ctx, &d_sel, sb));
-  LLFEM_HIP(hipcub::DeviceSelect::Flagged(d_sel, sb, ids, flags, m->d_multi, d_num, (int)m->nadj, st));
-  int64_t nm = 0;
-  LLFEM_HIP(hipMemcpyAsync(&nm, d_num, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  LLFEM_HIP(hipGetLastError());
-  LLFEM_HIP(hipStreamSynchronize(st));
-  dfree(ctx, d_sel, sb);
-  drelease(ctx, flags, (size_t)m->nadj);
-  drelease(ctx, d_num, 1);
-  drelease(ctx, node, (size_t)m->ninc);
-  drelease(ctx, keys, (size_t)n);
-  drelease(ctx, tmp, (size_t)n);
-  m->nmulti = nm;
-  m->ncontrib = n;
-  return LLFEM_OK;
+  int64_t *mref_of_blk = nullptr, *d_num = nullptr;
+  uint8_t* mflag = nullptr;
+  void *d_sort = nullptr, *d_scan = nullptr, *d_sel = nullptr;
+  size_t sort_b = 0, scan_b = 0, sel_b = 0;
+  auto cleanup = [&]() {
+    drelease(ctx, d_iperm, (size_t)npe);
+    drelease(ctx, node, (size_t)m->ninc);
+    drelease(ctx, flag, (size_t)n + 1);
+    drelease(ctx, sid, (size_t)n + 1);
+    drelease(ctx, mfirst, (size_t)m->nadj);
+    drelease(ctx, blk_node, (size_t)m->nadj);
+    drelease(ctx, multi, (size_t)m->nadj);
+    drelease(ctx, keys, (size_t)n);
+    drelease(ctx, tmp, (size_t)n);
+    drelease(ctx, mref_of_blk, 2 * (size_t)m->nadj);
+    drelease(ctx, d_num, 1);
+    drelease(ctx, mflag, (size_t)m->nadj);
+    if (d_sort) dfree(ctx, d_sort, sort_b);
+    if (d_scan) dfree(ctx, d_scan, scan_b);
+    if (d_sel) dfree(ctx, d_sel, sel_b);
+  };
+  auto run = [&]() -> int {
+    LLFEM_TRY(dalloc(ctx, &m->d_ho_perm, (size_t)npe));
+    LLFEM_TRY(dalloc(ctx, &d_iperm, (size_t)npe));
+    LLFEM_HIP(hipMemcpyAsync(m->d_ho_perm, perm.data(), sizeof(int32_t) * npe, hipMemcpyHostToDevice, st));
+    LLFEM_HIP(hipMemcpyAsync(d_iperm, iperm.data(), sizeof(int32_t) * npe, hipMemcpyHostToDevice, st));
+    LLFEM_TRY(dalloc(ctx, &node, (size_t)m->ninc));
+    LLFEM_TRY(dalloc(ctx, &keys, (size_t)n));
+    LLFEM_TRY(dalloc(ctx, &tmp, (size_t)n));
+    k_node_of_incidence<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_inc_ptr, node);
+    k_contrib_keys<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+        m->ninc, npe, node, m->d_inc_elem, m->d_inc_loc,
+        m->map_bytes == 1 ? (const uint8_t*)m->d_inc_map : nullptr,
+        m->map_bytes == 2 ? (const uint16_t*)m->d_inc_map : nullptr, m->d_adj_ptr, keys);
+    LLFEM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_b, keys, tmp, (int)n, 0, 64, st));
+    LLFEM_TRY(dmalloc(ctx, &d_sort, sort_b));
+    LLFEM_HIP(hipcub::DeviceRadixSort::SortKeys(d_sort, sort_b, keys, tmp, (int)n, 0, 64, st));
+    // staged ids: exclusive sum of the multi flags over the sorted contributions
+    LLFEM_TRY(dalloc(ctx, &flag, (size_t)n + 1));
+    LLFEM_TRY(dalloc(ctx, &sid, (size_t)n + 1));
+    k_multi_flags<<<(unsigned)((n + 256) / 256), 256, 0, st>>>(n, tmp, flag);
+    LLFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, flag, sid, (int)(n + 1), st));
+    LLFEM_TRY(dmalloc(ctx, &d_scan, scan_b));
+    LLFEM_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan, scan_b, flag, sid, (int)(n + 1), st));
+    LLFEM_TRY(dalloc(ctx, &blk_node, (size_t)m->nadj));
+    LLFEM_TRY(dalloc(ctx, &mfirst, (size_t)m->nadj));
+    LLFEM_TRY(dalloc(ctx, &mref_of_blk, 2 * (size_t)m->nadj));
+    LLFEM_TRY(dalloc(ctx, &m->d_gslot, (size_t)n));
+    k_blk_node<<<(unsigned)((m->N + 255) / 256), 256, 0, st>>>(m->N, m->d_adj_ptr, blk_node);
+    k_fill_i32<<<(unsigned)((m->nadj + 255) / 256), 256, 0, st>>>(m->nadj, -1, mfirst);
+    k_gslot<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, npe, tmp, flag, sid, d_iperm, blk_node, m->d_adj_ptr,
+                                                        m->d_gslot, mfirst, mref_of_blk);
+    int32_t ns = 0;
+    LLFEM_HIP(hipMemcpyAsync(&ns, sid + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    // multi blocks in ascending p
+    LLFEM_TRY(dalloc(ctx, &mflag, (size_t)m->nadj));
+    LLFEM_TRY(dalloc(ctx, &multi, (size_t)m->nadj));
+    LLFEM_TRY(dalloc(ctx, &d_num, 1));
+    k_mark_multi<<<(unsigned)((m->nadj + 255) / 256), 256, 0, st>>>(m->nadj, mfirst, mflag);
+    hipcub::CountingInputIterator<int32_t> ids(0);
+    LLFEM_HIP(hipcub::DeviceSelect::Flagged(nullptr, sel_b, ids, mflag, multi, d_num, (int)m->nadj, st));
+    LLFEM_TRY(dmalloc(ctx, &d_sel, sel_b));
+    LLFEM_HIP(hipcub::DeviceSelect::Flagged(d_sel, sel_b, ids, mflag, multi, d_num, (int)m->nadj, st));
+    int64_t nm = 0;
+    LLFEM_HIP(hipMemcpyAsync(&nm, d_num, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipStreamSynchronize(st));
+    m->nstaged = ns;
+    m->nmulti = nm;
+    LLFEM_TRY(dalloc(ctx, &m->d_mptr, (size_t)nm + 1));
+    LLFEM_TRY(dalloc(ctx, &m->d_mref, 2 * (size_t)nm));
+    k_multi_tables<<<(unsigned)((nm + 256) / 256), 256, 0, st>>>(nm, multi, mfirst, mref_of_blk, ns, m->d_mptr,
+                                                                 m->d_mref);
+    LLFEM_TRY(dalloc(ctx, &m->d_gcol, 2 * (size_t)m->E * npe));
+    k_gcol<<<(unsigned)((m->E * npe + 255) / 256), 256, 0, st>>>(m->E, npe, m->d_conn, m->d_ho_perm, m->d_adj_ptr,
+                                                                 m->d_gcol);
+    LLFEM_HIP(hipGetLastError());
+    LLFEM_HIP(hipStreamSynchronize(st));
+    return LLFEM_OK;
+  };
+  const int rc = run();
+  cleanup();
+  if (rc != LLFEM_OK) {
+    drelease(ctx, m->d_ho_perm, (size_t)npe);
+    drelease(ctx, m->d_gslot, (size_t)n);
+    drelease(ctx, m->d_gcol, 2 * (size_t)m->E * npe);
+    drelease(ctx, m->d_mptr, (size_t)m->nmulti + 1);
+    drelease(ctx, m->d_mref, 2 * (size_t)m->nmulti);
+    m->nmulti = m->nstaged = 0;
+  }
+  return rc;
 }
 
 // Stiffness of order >= 3 hexahedra (positive quadrature weights).  The
@@ -497,8 +636,9 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   const int rows = ntiles * kHoTile;
   const int qp = (q + 15) / 16 * 16;
   const size_t zd = (size_t)m->E * qp * rows;   // values per Z copy
-  const size_t sd = (size_t)m->E * npe * npe * 9;
-  const size_t need = (bf ? 2 * zd * sizeof(short) : zd * sizeof(double)) + sd * sizeof(double);
+  const size_t sd = (size_t)m->nstaged * 9;
+  const size_t sbytes = (sd * sizeof(double) + 255) / 256 * 256;
+  const size_t need = (bf ? 2 * zd * sizeof(short) : zd * sizeof(double)) + sbytes + 81 * sizeof(double);
   if (m->ho_bytes < need) {
     if (m->d_ho) dfree(ctx, m->d_ho, m->ho_bytes);
     m->d_ho = nullptr;
@@ -508,16 +648,11 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   }
   char* base = static_cast<char*>(m->d_ho);
   double* S = reinterpret_cast<double*>(base);
-  double* Z = reinterpret_cast<double*>(base + sd * sizeof(double));
+  double* Z = reinterpret_cast<double*>(base + sbytes);
   short* Zh = reinterpret_cast<short*>(Z);
   short* Zl = Zh + zd;
-  if (!m->d_ho_perm) {   // element 0's nodes in ascending global id
-    std::vector<int32_t> perm(npe);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return m->h_conn0[x] < m->h_conn0[y]; });
-    LLFEM_TRY(dalloc(ctx, &m->d_ho_perm, (size_t)npe));
-    LLFEM_HIP(hipMemcpyAsync(m->d_ho_perm, perm.data(), sizeof(int32_t) * npe, hipMemcpyHostToDevice, st));
-  }
+  double* dC = reinterpret_cast<double*>(base + sbytes + (bf ? 2 * zd * sizeof(short) : zd * sizeof(double)));
+  if (C81) LLFEM_HIP(hipMemcpyAsync(dC, C81, 81 * sizeof(double), hipMemcpyHostToDevice, st));
   if (bf)
     k_hex_z<true><<<(unsigned)m->E, qp, 0, st>>>(m->E, npe, q, rows, affine ? 1 : 0, m->d_geo, d_dN, d_w,
                                                  m->d_ho_perm, nullptr, Zh, Zl);
@@ -535,26 +670,22 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.Zl = Zl;
   P.lam = lam;
   P.mu = mu;
-  P.iso = C81 == nullptr;
-  if (C81) std::memcpy(P.C, C81, sizeof(P.C));
+  P.C = C81 ? dC : nullptr;
   P.S = S;
-  P.direct = m->d_direct;
-  P.perm = m->d_ho_perm;
-  P.blk_node = m->d_blk_node;
-  P.adj_ptr = m->d_adj_ptr;
+  P.gslot = m->d_gslot;
+  P.gcol = m->d_gcol;
   P.out = out;
   const size_t lds = sizeof(double) * 48 * kHoTile;
-  auto fn = bf ? k_gram_bf16x3 : k_gram_f64;
+  auto fn = bf ? (C81 ? k_gram_bf16x3<false> : k_gram_bf16x3<true>) : (C81 ? k_gram_f64<false> : k_gram_f64<true>);
   LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int64_t nblocks = (m->E * P.npairs + kHoXcds - 1) / kHoXcds * kHoXcds;
   if (nblocks >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "high-order path: too many elements");
   const dim3 grid((unsigned)nblocks);
   LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
-  hipLaunchKernelGGL(fn, grid, dim3(256), lds, st, P);
+  hipLaunchKernelGGL(fn, grid, dim3(kGramThreads), lds, st, P);
   LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
   if (m->nmulti > 0)
-    k_stage_gather<<<(unsigned)((m->nmulti + 255) / 256), 256, 0, st>>>(m->nmulti, m->d_multi, m->d_blk_node,
-                                                                         m->d_adj_ptr, m->d_cptr, m->d_csrc, S, out);
+    k_stage_gather<<<(unsigned)((m->nmulti + 255) / 256), 256, 0, st>>>(m->nmulti, m->d_mptr, m->d_mref, S, out);
   LLFEM_HIP(hipGetLastError());
   return LLFEM_OK;
 }

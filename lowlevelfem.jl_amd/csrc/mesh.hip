@@ -435,11 +435,10 @@ static void mesh_free(llfem_mesh* m) {
   drelease(c, m->d_lumped, 3 * m->N);
   drelease(c, m->d_valK32, 9 * m->nadj);
   drelease(c, m->d_diagK, 3 * m->N);
-  drelease(c, m->d_cptr, m->nadj + 1);
-  drelease(c, m->d_csrc, m->ncontrib);
-  drelease(c, m->d_blk_node, m->nadj);
-  drelease(c, m->d_direct, (size_t)m->E * m->npe * m->npe);
-  drelease(c, m->d_multi, (size_t)std::max<int64_t>(m->nmulti, 1));
+  drelease(c, m->d_gslot, (size_t)m->E * m->npe * m->npe);
+  drelease(c, m->d_gcol, 2 * (size_t)m->E * m->npe);
+  drelease(c, m->d_mptr, (size_t)m->nmulti + 1);
+  drelease(c, m->d_mref, 2 * (size_t)m->nmulti);
   if (m->d_ho) dfree(c, m->d_ho, m->ho_bytes);
   drelease(c, m->d_ho_perm, (size_t)m->npe);
   for (auto& kv : m->amg) {
