@@ -7,7 +7,9 @@
 //   Z[(b,al), p] = dN_b/dx_al (xi_p) * sqrt(w_p |det J_p|)      (3n x q)
 //   G = Z Z^T,   K_ba[d][c] = lam G_ba[d][c] + mu G_ba[c][d] + mu tr(G_ba) delta_dc
 // which is the same quadrature sum regrouped (2*(3n)^2*q flops per element
-// instead of 2*(3n)^2*6q).  G runs on MFMA:
+// instead of 2*(3n)^2*6q).  Z is never stored: each Gram block builds the Z
+// rows of its two tiles in LDS, 16 points at a time, from the element's
+// geometry and the reference gradients.  G runs on MFMA:
 //   FP64    v_mfma_f64_16x16x4f64,
 //   BF16x3  v_mfma_f32_16x16x16bf16_1k with Z split into bf16 hi + lo parts,
 //           G ~ Zh Zh^T + Zh Zl^T + Zl Zh^T, FP32 accumulation.
@@ -23,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "common.h"
 
@@ -49,52 +52,19 @@ __device__ __forceinline__ float bf16_to_f(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
-// Z for one (element, point); rows = padded 3n (multiple of 96), qp = padded
-// q (multiple of 16).  Layouts:
-//   FP64  Z[e][p/4][row][p%4]           (k-blocks of 4: one MFMA fragment = 512 contiguous B)
-//   BF16  Zh/Zl[e][p/16][row][p%16]     (k-blocks of 16)
-// Padding rows (>= 3n) and points (>= q) are written as zero.
-// Row position b holds local node perm[b] (the element's nodes in ascending
-// global node id, so the Gram epilogue's neighbouring lanes write
-// neighbouring CSC slots).
-template <bool BF16>
-__global__ void k_hex_z(int64_t E, int npe, int q, int rows, int affine, const double* __restrict__ geo,
-                        const double* __restrict__ dN, const double* __restrict__ w,
-                        const int32_t* __restrict__ perm,
-                        double* __restrict__ Z, short* __restrict__ Zh, short* __restrict__ Zl) {
-  const int64_t e = blockIdx.x;
-  const int p = threadIdx.x;           // point (blockDim = qp)
-  const int qp = blockDim.x;
-  double Mi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  double sw = 0.0;
-  if (p < q) {
-    const double* g = geo + (affine ? (size_t)e : ((size_t)e * q + p)) * kGeoRec;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) Mi[m] = g[m];
-    sw = sqrt(affine ? g[9] * w[p] : g[9]);
-  }
-  for (int b = 0; b < rows / 3; ++b) {
-    double z[3] = {0.0, 0.0, 0.0};
-    if (p < q && b < npe) {
-      const double* h = dN + ((size_t)p * npe + perm[b]) * 3;
-#pragma unroll
-      for (int al = 0; al < 3; ++al) z[al] = sw * (Mi[al] * h[0] + Mi[3 + al] * h[1] + Mi[6 + al] * h[2]);
-    }
-#pragma unroll
-    for (int al = 0; al < 3; ++al) {
-      const int row = 3 * b + al;
-      if constexpr (!BF16) {
-        Z[(((size_t)e * (qp / 4) + p / 4) * rows + row) * 4 + (p & 3)] = z[al];
-      } else {
-        const float f = (float)z[al];
-        const short hi = bf16_rn(f);
-        const short lo = bf16_rn((float)(z[al] - (double)bf16_to_f(hi)));
-        const size_t o = (((size_t)e * (qp / 16) + p / 16) * rows + row) * 16 + (p & 15);
-        Zh[o] = hi;
-        Zl[o] = lo;
-      }
-    }
-  }
+// Reference gradients in Gram row-position order, padded: dNp[p][pos][c] =
+// dN_{perm[pos]}/dxi_c at point p, zero for padding points (p >= q) and
+// positions (pos >= npe).  Row position pos holds local node perm[pos] (the
+// element's nodes in ascending global node id, so the Gram epilogue's
+// neighbouring lanes write neighbouring CSC slots).
+__global__ void k_dn_positions(int q, int qp, int npe, int npos, const int32_t* __restrict__ perm,
+                               const double* __restrict__ dN, double* __restrict__ dNp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)qp * npos * 3) return;
+  const int c = (int)(i % 3);
+  const int pos = (int)((i / 3) % npos);
+  const int p = (int)(i / (3 * npos));
+  dNp[i] = (p < q && pos < npe) ? dN[((size_t)p * npe + perm[pos]) * 3 + c] : 0.0;
 }
 
 constexpr int kGramThreads = 256;   // 4 waves; the epilogue deals 2 x 256 blocks per pass
@@ -104,10 +74,12 @@ struct GramParams {
   int npe;
   int ntiles;             // node tiles of 32 (rows = 96 * ntiles)
   int npairs;             // upper tile pairs per element
-  int qp;                 // padded points
-  const double* Z;
-  const short* Zh;
-  const short* Zl;
+  int q, qp;              // points, padded to a multiple of 16
+  int npos;               // row positions (32 per tile)
+  int affine;             // one geometry record per element (else per point)
+  const double* geo;      // geometry records: inverse Jacobian [9], |det J| (affine) or w |det J|
+  const double* w;        // [q] quadrature weights
+  const double* dNp;      // [qp][npos][3] reference gradients by row position
   double lam, mu;
   const double* C;        // general material [d][x][c][y] in device memory (NULL: isotropic lam, mu)
   double* S;              // staged blocks [k][c][d]
@@ -228,77 +200,57 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, const int64
   }
 }
 
-// One block per (element, upper tile pair); 4 waves, each a 3x3 grid of 16x16
-// subtiles of the 96x96 tile (wave w: rows 48*(w/2).., cols 48*(w%2)..).
-// Fragments of step kb+1 are loaded while the MFMAs of step kb run.  The tile
-// goes through LDS in two 48-row halves (36 KB), so four blocks fit a CU.
-template <bool ISO>
-__global__ __launch_bounds__(kGramThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
-void k_gram_f64(GramParams P) {
-  extern __shared__ double Gt[];   // 48 x 96
-  __shared__ int64_t colref[2 * 64];
-  __shared__ double Cs[ISO ? 1 : 81];
-  if constexpr (!ISO)
-    for (int i = threadIdx.x; i < 81; i += blockDim.x) Cs[i] = P.C[i];
-  int64_t e;
-  int ti, tj;
-  if (!gram_block(P, e, ti, tj)) return;   // block-uniform
-  load_colref(P, e, ti, tj, colref);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rows = P.ntiles * kHoTile;
-  const int lr = lane & 15, lk = lane >> 4;
-  const double* Za = P.Z + (size_t)e * P.qp * rows + (size_t)(ti * kHoTile + (wave >> 1) * 48 + lr) * 4 + lk;
-  const double* Zb = P.Z + (size_t)e * P.qp * rows + (size_t)(tj * kHoTile + (wave & 1) * 48 + lr) * 4 + lk;
-  const size_t kstride = (size_t)rows * 4;
-  const int nk = P.qp / 4;   // even: qp is a multiple of 16
-  d4 acc[3][3];
+// Z rows of the chunk's 16 points, built in LDS.  Per point: the inverse
+// Jacobian and sqrt(w |det J|) (pm); per (tile, row position, point): the
+// three weighted physical gradients
+//   z[al] = sw * (Mi[al] h0 + Mi[3+al] h1 + Mi[6+al] h2),  h = dN/dxi.
+// FP64 layout: zs[tile][p][row] with row stride kZrs (the f64 MFMA reads a
+// 16-row x 4-point fragment); BF16: hi and lo parts zh/zl[tile][row][p]
+// (the bf16 MFMA reads 4 consecutive points of a row).
+constexpr int kZpts = 16;     // points per chunk
+constexpr int kZrs = 112;     // FP64 row stride of a point's Z column (96 rows + pad: 2 LDS cycles per fragment)
+
+template <bool BF16>
+__device__ __forceinline__ void gram_build_z(const GramParams& P, const double* pm, int ti, int tj, int c0,
+                                             double* zs, short* zh, short* zl) {
+  const int ntile = ti == tj ? 1 : 2;
+  for (int f = threadIdx.x; f < ntile * 32 * kZpts; f += kGramThreads) {
+    const int tile = f / (32 * kZpts);
+    const int p = (f / 32) % kZpts, n = f % 32;
+    const int pos = (tile == 0 ? ti : tj) * 32 + n;
+    const double* h = P.dNp + ((size_t)(c0 + p) * P.npos + pos) * 3;
+    const double h0 = h[0], h1 = h[1], h2 = h[2];
+    const double* M = pm + p * 10;
+    const double sw = M[9];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  double a0[3], b0[3], a1[3], b1[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) { a0[i] = Za[64 * i]; b0[i] = Zb[64 * i]; }
-  for (int kb = 0; kb < nk; kb += 2) {
-    const size_t o1 = (size_t)(kb + 1) * kstride;
-    const size_t o2 = (size_t)(kb + 2 < nk ? kb + 2 : nk - 1) * kstride;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { a1[i] = Za[o1 + 64 * i]; b1[i] = Zb[o1 + 64 * i]; }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i], b0[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { a0[i] = Za[o2 + 64 * i]; b0[i] = Zb[o2 + 64 * i]; }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i], b1[j], acc[i][j], 0, 0, 0);
-  }
-  // f64 16x16x4 output layout: lane l, reg r -> row (l/16) + 4r, col l%16
-  const int tc0 = (wave & 1) * 48;
-  for (int h = 0; h < 2; ++h) {
-    if ((wave >> 1) == h) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            Gt[(16 * i + lk + 4 * r) * kHoTile + tc0 + 16 * j + lr] = acc[i][j][r];
+    for (int al = 0; al < 3; ++al) {
+      const double z = sw * (M[al] * h0 + M[3 + al] * h1 + M[6 + al] * h2);
+      const int row = 3 * n + al;
+      if constexpr (!BF16) {
+        zs[(tile * kZpts + p) * kZrs + row] = z;
+      } else {
+        const short hi = bf16_rn((float)z);
+        const short lo = bf16_rn((float)(z - (double)bf16_to_f(hi)));
+        zh[(tile * kHoTile + row) * kZpts + p] = hi;
+        zl[(tile * kHoTile + row) * kZpts + p] = lo;
+      }
     }
-    __syncthreads();
-    gram_epilogue<ISO>(P, Gt, colref, Cs, e, ti, tj, h);
-    __syncthreads();
   }
 }
 
-template <bool ISO>
+// One block per (element, upper tile pair); 4 waves, each a 3x3 grid of 16x16
+// subtiles of the 96x96 tile (wave w: rows 48*(w/2).., cols 48*(w%2)..).
+// Per chunk of 16 points: per-point geometry into LDS, the two tiles' Z rows
+// into LDS, then the MFMAs of the chunk read their fragments from LDS.  The
+// tile then goes through the same LDS in two 48-row halves (36 KB), so four
+// blocks fit a CU.
+template <bool BF16, bool ISO>
 __global__ __launch_bounds__(kGramThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
-void k_gram_bf16x3(GramParams P) {
-  extern __shared__ double Gt[];   // 48 x 96
+void k_gram(GramParams P) {
+  extern __shared__ double Gt[];   // 48 x 96 doubles; during the k loop the Z chunk
   __shared__ int64_t colref[2 * 64];
   __shared__ double Cs[ISO ? 1 : 81];
+  __shared__ double pm[kZpts * 10];
   if constexpr (!ISO)
     for (int i = threadIdx.x; i < 81; i += blockDim.x) Cs[i] = P.C[i];
   int64_t e;
@@ -306,51 +258,72 @@ void k_gram_bf16x3(GramParams P) {
   if (!gram_block(P, e, ti, tj)) return;   // block-uniform
   load_colref(P, e, ti, tj, colref);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rows = P.ntiles * kHoTile;
   const int lr = lane & 15, lk = lane >> 4;
-  const size_t eo = (size_t)e * P.qp * rows;
-  const size_t oa = eo + (size_t)(ti * kHoTile + (wave >> 1) * 48 + lr) * 16 + 4 * lk;
-  const size_t ob = eo + (size_t)(tj * kHoTile + (wave & 1) * 48 + lr) * 16 + 4 * lk;
-  const s4* Ah = reinterpret_cast<const s4*>(P.Zh + oa);
-  const s4* Al = reinterpret_cast<const s4*>(P.Zl + oa);
-  const s4* Bh = reinterpret_cast<const s4*>(P.Zh + ob);
-  const s4* Bl = reinterpret_cast<const s4*>(P.Zl + ob);
-  const size_t kstride = (size_t)rows * 4;   // in s4 units (16 shorts per row per k-block)
-  const int nk = P.qp / 16;
-  f4 acc[3][3];
+  const int tb = ti == tj ? 0 : 1;        // tile of the B operand in the chunk buffers
+  const int ra = (wave >> 1) * 48 + lr, rb = (wave & 1) * 48 + lr;
+  double* zs = Gt;
+  short* zh = reinterpret_cast<short*>(Gt);
+  short* zl = zh + 2 * kHoTile * kZpts;
+  using Acc = typename std::conditional<BF16, f4, d4>::type;
+  Acc acc[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  s4 ah[2][3], al[2][3], bh[2][3], bl[2][3];
+    for (int j = 0; j < 3; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+  for (int c0 = 0; c0 < P.qp; c0 += kZpts) {
+    // per-point geometry of the chunk (the previous chunk's MFMAs are done:
+    // barrier at the end of the loop body)
+    if (threadIdx.x < kZpts * 10) {
+      const int p = threadIdx.x / 10, m = threadIdx.x % 10;
+      const int pg = c0 + p;
+      double v = 0.0;
+      if (pg < P.q) {
+        const double* g = P.geo + (P.affine ? (size_t)e : ((size_t)e * P.q + pg)) * kGeoRec;
+        v = m < 9 ? g[m] : sqrt(P.affine ? g[9] * P.w[pg] : g[9]);
+      }
+      pm[threadIdx.x] = v;
+    }
+    __syncthreads();
+    gram_build_z<BF16>(P, pm, ti, tj, c0, zs, zh, zl);
+    __syncthreads();
+    if constexpr (!BF16) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    ah[0][i] = Ah[64 * i]; al[0][i] = Al[64 * i];
-    bh[0][i] = Bh[64 * i]; bl[0][i] = Bl[64 * i];
-  }
-  for (int kb = 0; kb < nk; ++kb) {
-    // nk = qp/16 can be odd: the buffer index is kb & 1, resolved by unrolling two steps
+      for (int kb = 0; kb < kZpts / 4; ++kb) {
+        double a[3], b[3];
+        const double* za = zs + (kb * 4 + lk) * kZrs;
+        const double* zb = zs + (tb * kZpts + kb * 4 + lk) * kZrs;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (u == 1) { if (++kb >= nk) break; }
-      const int cur = u, nxt = u ^ 1;
-      const size_t on = (size_t)(kb + 1 < nk ? kb + 1 : nk - 1) * kstride;
+        for (int i = 0; i < 3; ++i) {
+          a[i] = za[ra + 16 * i];
+          b[i] = zb[rb + 16 * i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    } else {
+      s4 ah[3], al_[3], bh[3], bl[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        ah[nxt][i] = Ah[on + 64 * i]; al[nxt][i] = Al[on + 64 * i];
-        bh[nxt][i] = Bh[on + 64 * i]; bl[nxt][i] = Bl[on + 64 * i];
+        ah[i] = *reinterpret_cast<const s4*>(zh + (ra + 16 * i) * kZpts + 4 * lk);
+        al_[i] = *reinterpret_cast<const s4*>(zl + (ra + 16 * i) * kZpts + 4 * lk);
+        bh[i] = *reinterpret_cast<const s4*>(zh + (tb * kHoTile + rb + 16 * i) * kZpts + 4 * lk);
+        bl[i] = *reinterpret_cast<const s4*>(zl + (tb * kHoTile + rb + 16 * i) * kZpts + 4 * lk);
       }
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[cur][i], bh[cur][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[cur][i], bl[cur][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[cur][i], bh[cur][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al_[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
+    __syncthreads();
   }
-  // bf16 16x16x16 output layout: lane l, reg r -> row 4(l/16) + r, col l%16
+  // output layouts: f64 16x16x4: lane l, reg r -> row (l/16) + 4r, col l%16;
+  // bf16 16x16x16: row 4(l/16) + r, col l%16
   const int tc0 = (wave & 1) * 48;
   for (int h = 0; h < 2; ++h) {
     if ((wave >> 1) == h) {
@@ -359,8 +332,10 @@ void k_gram_bf16x3(GramParams P) {
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            Gt[(16 * i + 4 * lk + r) * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
+          for (int r = 0; r < 4; ++r) {
+            const int row = BF16 ? 16 * i + 4 * lk + r : 16 * i + lk + 4 * r;
+            Gt[row * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
+          }
     }
     __syncthreads();
     gram_epilogue<ISO>(P, Gt, colref, Cs, e, ti, tj, h);
@@ -633,12 +608,14 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   LLFEM_TRY(build_contrib(m));
   const bool bf = d->precision == LLFEM_BF16X3;
   const int ntiles = (3 * npe + kHoTile - 1) / kHoTile;
-  const int rows = ntiles * kHoTile;
-  const int qp = (q + 15) / 16 * 16;
-  const size_t zd = (size_t)m->E * qp * rows;   // values per Z copy
-  const size_t sd = (size_t)m->nstaged * 9;
-  const size_t sbytes = (sd * sizeof(double) + 255) / 256 * 256;
-  const size_t need = (bf ? 2 * zd * sizeof(short) : zd * sizeof(double)) + sbytes + 81 * sizeof(double);
+  const int npos = 32 * ntiles;
+  const int qp = (q + kZpts - 1) / kZpts * kZpts;
+  auto al256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  // d_ho: staged blocks | general material | reference gradients by row position
+  const size_t sbytes = al256((size_t)m->nstaged * 9 * sizeof(double));
+  const size_t cbytes = al256(81 * sizeof(double));
+  const size_t nbytes = al256((size_t)qp * npos * 3 * sizeof(double));
+  const size_t need = sbytes + cbytes + nbytes;
   if (m->ho_bytes < need) {
     if (m->d_ho) dfree(ctx, m->d_ho, m->ho_bytes);
     m->d_ho = nullptr;
@@ -648,26 +625,23 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   }
   char* base = static_cast<char*>(m->d_ho);
   double* S = reinterpret_cast<double*>(base);
-  double* Z = reinterpret_cast<double*>(base + sbytes);
-  short* Zh = reinterpret_cast<short*>(Z);
-  short* Zl = Zh + zd;
-  double* dC = reinterpret_cast<double*>(base + sbytes + (bf ? 2 * zd * sizeof(short) : zd * sizeof(double)));
+  double* dC = reinterpret_cast<double*>(base + sbytes);
+  double* dNp = reinterpret_cast<double*>(base + sbytes + cbytes);
   if (C81) LLFEM_HIP(hipMemcpyAsync(dC, C81, 81 * sizeof(double), hipMemcpyHostToDevice, st));
-  if (bf)
-    k_hex_z<true><<<(unsigned)m->E, qp, 0, st>>>(m->E, npe, q, rows, affine ? 1 : 0, m->d_geo, d_dN, d_w,
-                                                 m->d_ho_perm, nullptr, Zh, Zl);
-  else
-    k_hex_z<false><<<(unsigned)m->E, qp, 0, st>>>(m->E, npe, q, rows, affine ? 1 : 0, m->d_geo, d_dN, d_w,
-                                                  m->d_ho_perm, Z, nullptr, nullptr);
+  const int64_t ndn = (int64_t)qp * npos * 3;
+  k_dn_positions<<<(unsigned)((ndn + 255) / 256), 256, 0, st>>>(q, qp, npe, npos, m->d_ho_perm, d_dN, dNp);
   GramParams P{};
   P.E = m->E;
   P.npe = npe;
   P.ntiles = ntiles;
   P.npairs = ntiles * (ntiles + 1) / 2;
+  P.q = q;
   P.qp = qp;
-  P.Z = Z;
-  P.Zh = Zh;
-  P.Zl = Zl;
+  P.npos = npos;
+  P.affine = affine ? 1 : 0;
+  P.geo = m->d_geo;
+  P.w = d_w;
+  P.dNp = dNp;
   P.lam = lam;
   P.mu = mu;
   P.C = C81 ? dC : nullptr;
@@ -676,7 +650,7 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.gcol = m->d_gcol;
   P.out = out;
   const size_t lds = sizeof(double) * 48 * kHoTile;
-  auto fn = bf ? (C81 ? k_gram_bf16x3<false> : k_gram_bf16x3<true>) : (C81 ? k_gram_f64<false> : k_gram_f64<true>);
+  auto fn = bf ? (C81 ? k_gram<true, false> : k_gram<true, true>) : (C81 ? k_gram<false, false> : k_gram<false, true>);
   LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int64_t nblocks = (m->E * P.npairs + kHoXcds - 1) / kHoXcds * kHoXcds;
   if (nblocks >= ((int64_t)1 << 31)) return fail(LLFEM_EUNSUP, "high-order path: too many elements");
