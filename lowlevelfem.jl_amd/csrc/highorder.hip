@@ -52,22 +52,26 @@ __device__ __forceinline__ float bf16_to_f(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
-// Reference gradients in Gram row-position order, padded: dNp[p][pos][c] =
-// dN_{perm[pos]}/dxi_c at point p, zero for padding points (p >= q) and
-// positions (pos >= npe).  Row position pos holds local node perm[pos] (the
-// element's nodes in ascending global node id, so the Gram epilogue's
-// neighbouring lanes write neighbouring CSC slots).
-__global__ void k_dn_positions(int q, int qp, int npe, int npos, const int32_t* __restrict__ perm,
+// Reference gradients in Gram row-position order, padded: dN_{perm[pos]}/dxi_c
+// at point p, zero for padding points (p >= q) and positions (pos >= npe),
+// stored [p][pos][c] (points_major) or [pos][p][c].  Row position pos holds
+// local node perm[pos] (the element's nodes in ascending global node id, so
+// the Gram epilogue's neighbouring lanes write neighbouring CSC slots).
+__global__ void k_dn_positions(int q, int qp, int npe, int npos, int points_major, const int32_t* __restrict__ perm,
                                const double* __restrict__ dN, double* __restrict__ dNp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)qp * npos * 3) return;
   const int c = (int)(i % 3);
-  const int pos = (int)((i / 3) % npos);
-  const int p = (int)(i / (3 * npos));
+  const int pos = points_major ? (int)((i / 3) % npos) : (int)(i / (3 * qp));
+  const int p = points_major ? (int)(i / (3 * npos)) : (int)((i / 3) % qp);
   dNp[i] = (p < q && pos < npe) ? dN[((size_t)p * npe + perm[pos]) * 3 + c] : 0.0;
 }
 
 constexpr int kGramThreads = 256;   // 4 waves; the epilogue deals 2 x 256 blocks per pass
+// Row stride (doubles) of the G half-tile in LDS.  The epilogue's lanes read
+// rows 3 apart (pass 0) or columns 3 apart (pass 1); with 100 = 4 (mod 32)
+// both patterns touch each 8-byte bank pair at most twice per wavefront.
+constexpr int kGtStride = 100;
 
 struct GramParams {
   int64_t E;
@@ -79,7 +83,7 @@ struct GramParams {
   int affine;             // one geometry record per element (else per point)
   const double* geo;      // geometry records: inverse Jacobian [9], |det J| (affine) or w |det J|
   const double* w;        // [q] quadrature weights
-  const double* dNp;      // [qp][npos][3] reference gradients by row position
+  const double* dNp;      // reference gradients by row position: FP64 [qp][npos][3], BF16 [npos][qp][3]
   double lam, mu;
   const double* C;        // general material [d][x][c][y] in device memory (NULL: isotropic lam, mu)
   double* S;              // staged blocks [k][c][d]
@@ -157,7 +161,7 @@ __device__ void gram_epilogue(const GramParams& P, const double* Gt, const int64
     for (int x = 0; x < 3; ++x)
 #pragma unroll
       for (int y = 0; y < 3; ++y) {
-        const double g = Gt[(3 * hl + x) * kHoTile + 3 * al + y];
+        const double g = Gt[(3 * hl + x) * kGtStride + 3 * al + y];
         if (pass == 0) G[x * 3 + y] = g; else G[y * 3 + x] = g;
       }
     // destination of the block of (column node, row node): [c][d] at o[c * len + d]
@@ -216,9 +220,12 @@ __device__ __forceinline__ void gram_build_z(const GramParams& P, const double* 
   const int ntile = ti == tj ? 1 : 2;
   for (int f = threadIdx.x; f < ntile * 32 * kZpts; f += kGramThreads) {
     const int tile = f / (32 * kZpts);
-    const int p = (f / 32) % kZpts, n = f % 32;
+    // FP64: row positions fastest across lanes (the zs column of a point is
+    // contiguous); BF16: points fastest (a row's 16 points are contiguous)
+    const int p = BF16 ? f % kZpts : (f / 32) % kZpts;
+    const int n = BF16 ? (f / kZpts) % 32 : f % 32;
     const int pos = (tile == 0 ? ti : tj) * 32 + n;
-    const double* h = P.dNp + ((size_t)(c0 + p) * P.npos + pos) * 3;
+    const double* h = BF16 ? P.dNp + ((size_t)pos * P.qp + c0 + p) * 3 : P.dNp + ((size_t)(c0 + p) * P.npos + pos) * 3;
     const double h0 = h[0], h1 = h[1], h2 = h[2];
     const double* M = pm + p * 10;
     const double sw = M[9];
@@ -247,7 +254,7 @@ __device__ __forceinline__ void gram_build_z(const GramParams& P, const double* 
 template <bool BF16, bool ISO>
 __global__ __launch_bounds__(kGramThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_gram(GramParams P) {
-  extern __shared__ double Gt[];   // 48 x 96 doubles; during the k loop the Z chunk
+  extern __shared__ double Gt[];   // 48 rows x kGtStride doubles; during the k loop the Z chunk
   __shared__ int64_t colref[2 * 64];
   __shared__ double Cs[ISO ? 1 : 81];
   __shared__ double pm[kZpts * 10];
@@ -334,7 +341,7 @@ void k_gram(GramParams P) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = BF16 ? 16 * i + 4 * lk + r : 16 * i + lk + 4 * r;
-            Gt[row * kHoTile + tc0 + 16 * j + lr] = (double)acc[i][j][r];
+            Gt[row * kGtStride + tc0 + 16 * j + lr] = (double)acc[i][j][r];
           }
     }
     __syncthreads();
@@ -343,25 +350,19 @@ void k_gram(GramParams P) {
   }
 }
 
-// CSC blocks with more than one contribution: thread per block, its staged
-// blocks are contiguous (ascending element order) and summed in that order.
+// CSC blocks with more than one contribution: nine lanes per block, one per
+// entry [c][d]; its staged blocks are contiguous (ascending element order)
+// and summed in that order, so neighbouring lanes read neighbouring doubles.
 __global__ void k_stage_gather(int64_t nmulti, const int32_t* __restrict__ mptr, const int64_t* __restrict__ mref,
                                const double* __restrict__ S, double* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nmulti) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nmulti * 9) return;
+  const int64_t k = t / 9;
+  const int m = (int)(t - 9 * k);
   const int64_t c0 = mptr[k], c1 = mptr[k + 1];
-  const int64_t base = mref[2 * k], len = mref[2 * k + 1];
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t c = c0; c < c1; ++c) {
-    const double* blk = S + (size_t)c * 9;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) acc[m] += blk[m];
-  }
-  double* o = out + base;
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) o[c * len + d] = acc[c * 3 + d];
+  double acc = 0.0;
+  for (int64_t c = c0; c < c1; ++c) acc += S[(size_t)c * 9 + m];
+  out[mref[2 * k] + (m / 3) * mref[2 * k + 1] + m % 3] = acc;
 }
 
 // contribution keys: (block p << 32) | element block e*npe^2 + a*npe + b
@@ -629,7 +630,8 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   double* dNp = reinterpret_cast<double*>(base + sbytes + cbytes);
   if (C81) LLFEM_HIP(hipMemcpyAsync(dC, C81, 81 * sizeof(double), hipMemcpyHostToDevice, st));
   const int64_t ndn = (int64_t)qp * npos * 3;
-  k_dn_positions<<<(unsigned)((ndn + 255) / 256), 256, 0, st>>>(q, qp, npe, npos, m->d_ho_perm, d_dN, dNp);
+  k_dn_positions<<<(unsigned)((ndn + 255) / 256), 256, 0, st>>>(q, qp, npe, npos, bf ? 0 : 1, m->d_ho_perm, d_dN,
+                                                                 dNp);
   GramParams P{};
   P.E = m->E;
   P.npe = npe;
@@ -649,7 +651,7 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   P.gslot = m->d_gslot;
   P.gcol = m->d_gcol;
   P.out = out;
-  const size_t lds = sizeof(double) * 48 * kHoTile;
+  const size_t lds = sizeof(double) * 48 * kGtStride;
   auto fn = bf ? (C81 ? k_gram<true, false> : k_gram<true, true>) : (C81 ? k_gram<false, false> : k_gram<false, true>);
   LLFEM_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int64_t nblocks = (m->E * P.npairs + kHoXcds - 1) / kHoXcds * kHoXcds;
@@ -659,7 +661,8 @@ int assemble_highorder(llfem_mesh* m, const llfem_op_desc* d, bool affine, const
   hipLaunchKernelGGL(fn, grid, dim3(kGramThreads), lds, st, P);
   LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
   if (m->nmulti > 0)
-    k_stage_gather<<<(unsigned)((m->nmulti + 255) / 256), 256, 0, st>>>(m->nmulti, m->d_mptr, m->d_mref, S, out);
+    k_stage_gather<<<(unsigned)((9 * m->nmulti + 255) / 256), 256, 0, st>>>(m->nmulti, m->d_mptr, m->d_mref, S,
+                                                                             out);
   LLFEM_HIP(hipGetLastError());
   return LLFEM_OK;
 }
