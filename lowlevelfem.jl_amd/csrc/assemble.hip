@@ -294,9 +294,11 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const doubl
     double* dNT = m->d_tab + dnt_offset(npe, q);
     const int64_t nt = (int64_t)q * npe * 3;
     k_dn_transpose<<<(unsigned)((nt + 255) / 256), 256, 0, ctx->stream>>>(npe, q, m->d_tab, dNT);
-    k_geometry_curved<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
-        m->E, m->N, npe, q, m->d_xyz, m->d_conn, dNT, m->d_tab + (size_t)q * npe * 3, m->d_geo, d_scale);
-    return LLFEM_OK;
+    if (npe <= kCurvedMaxNpe) {
+      k_geometry_curved<<<(unsigned)m->E, 128, 0, ctx->stream>>>(m->E, m->N, npe, q, m->d_xyz, m->d_conn, dNT,
+                                                                 m->d_tab + (size_t)q * npe * 3, m->d_geo, d_scale);
+      return LLFEM_OK;
+    }
   }
   k_geometry<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(
       m->E, m->N, npe, q, affine ? 1 : 0, m->d_xyz, m->d_conn, m->d_tab,

@@ -360,9 +360,19 @@ __global__ void k_stage_gather(int64_t nmulti, const int32_t* __restrict__ mptr,
   const int64_t k = t / 9;
   const int m = (int)(t - 9 * k);
   const int64_t c0 = mptr[k], c1 = mptr[k + 1];
+  const int64_t base = mref[2 * k], len = mref[2 * k + 1];
+  // eight contributions per round, all loads issued before the sums (a
+  // hexahedral mesh has at most eight elements at a node pair)
   double acc = 0.0;
-  for (int64_t c = c0; c < c1; ++c) acc += S[(size_t)c * 9 + m];
-  out[mref[2 * k] + (m / 3) * mref[2 * k + 1] + m % 3] = acc;
+  for (int64_t c = c0; c < c1; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = c + i < c1 ? S[(size_t)(c + i) * 9 + m] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (c + i < c1) acc += v[i];
+  }
+  out[base + (m / 3) * len + m % 3] = acc;
 }
 
 // contribution keys: (block p << 32) | element block e*npe^2 + a*npe + b

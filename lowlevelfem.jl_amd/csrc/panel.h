@@ -170,36 +170,45 @@ __global__ __launch_bounds__(256) void k_dn_transpose(int npe, int q, const doub
   dNT[t] = dN[((int64_t)p * npe + a) * 3 + c];
 }
 
-// Curved elements: thread per (element, point), points fastest, the same sums
-// in the same order as jacobian(); the gradients come from dNT, so the lanes
-// of a wave (neighbouring points) load neighbouring doubles instead of rows
-// npe * 24 bytes apart.
-__global__ __launch_bounds__(256) void k_geometry_curved(int64_t E, int64_t N, int npe, int q,
+// Curved elements: one block per element, thread per point (blockDim >= q).
+// The element's node coordinates, relative to its first node, are gathered
+// once into LDS; each point then sums over the nodes with the same sums in
+// the same order as jacobian(), its gradients from dNT (the lanes of a wave,
+// neighbouring points, load neighbouring doubles).
+constexpr int kCurvedMaxNpe = 128;
+
+__global__ __launch_bounds__(128) void k_geometry_curved(int64_t E, int64_t N, int npe, int q,
                                                          const double* __restrict__ xyz,
                                                          const int32_t* __restrict__ conn,
                                                          const double* __restrict__ dNT,
                                                          const double* __restrict__ w, double* __restrict__ geo,
                                                          const double* __restrict__ scale) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= E * q) return;
-  const int64_t e = t / q;
-  const int p = (int)(t - e * q);
-  double J[9], Mi[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+  __shared__ double rel[3][kCurvedMaxNpe];
+  const int64_t e = blockIdx.x;
   const int32_t v0 = conn[e];
   const double x0 = xyz[v0], y0 = xyz[N + v0], z0 = xyz[2 * N + v0];
-  for (int a = 1; a < npe; ++a) {
+  for (int a = threadIdx.x; a < npe; a += blockDim.x) {
     const int32_t v = conn[(int64_t)a * E + e];
-    const double x = xyz[v] - x0, y = xyz[N + v] - y0, z = xyz[2 * N + v] - z0;
-    const double* g = dNT + ((int64_t)a * q + p) * 3;
-    const double g0 = g[0], g1 = g[1], g2 = g[2];
-    J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
-    J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
-    J[6] += z * g0; J[7] += z * g1; J[8] += z * g2;
+    rel[0][a] = xyz[v] - x0;
+    rel[1][a] = xyz[N + v] - y0;
+    rel[2][a] = xyz[2 * N + v] - z0;
   }
-  const double det = inv3(J, Mi);
-  geometry_store(geo + (size_t)t * kGeoStride, Mi, fabs(det) * w[p] * (scale ? scale[e] : 1.0));
+  __syncthreads();
+  for (int p = threadIdx.x; p < q; p += blockDim.x) {
+    double J[9], Mi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) J[k] = 0.0;
+    for (int a = 1; a < npe; ++a) {
+      const double x = rel[0][a], y = rel[1][a], z = rel[2][a];
+      const double* g = dNT + ((int64_t)a * q + p) * 3;
+      const double g0 = g[0], g1 = g[1], g2 = g[2];
+      J[0] += x * g0; J[1] += x * g1; J[2] += x * g2;
+      J[3] += y * g0; J[4] += y * g1; J[5] += y * g2;
+      J[6] += z * g0; J[7] += z * g1; J[8] += z * g2;
+    }
+    const double det = inv3(J, Mi);
+    geometry_store(geo + ((size_t)e * q + p) * kGeoStride, Mi, fabs(det) * w[p] * (scale ? scale[e] : 1.0));
+  }
 }
 
 // Affine: thread per element.  Curved: thread per (element, point), so
