@@ -39,38 +39,72 @@ def hex_prism_box(cells=(4, 3, 4), lx=1.0, ly=0.8, lz=1.2, split=None):
                   blocks=[M.Block(6, pconn, pvol)])
 
 
-def write_msh(mesh, path):
-    """MSH 4.1 ASCII: one volume entity per (material group, element type),
-    one surface entity per (face group, face type); node tags 1..N."""
-    groups = ["steel", "alu"]
+def _gmsh_conn(etype, conn):
+    """Element-local node order -> gmsh's (high-order hexahedra/quadrangles)."""
+    from lowlevelfem.msh import GMSH_PERMUTED, gmsh_to_local
+    if etype in GMSH_PERMUTED:
+        return conn[:, np.argsort(gmsh_to_local(etype))]
+    return conn
+
+
+def write_msh(mesh, path, binary=False):
+    """MSH 4.1, ASCII or binary (little-endian, data size 8): one volume
+    entity per (volume group, element type), one surface entity per (face
+    group, face type); node tags 1..N; element nodes in gmsh's order."""
+    groups = list(dict.fromkeys(g for b in mesh.element_blocks() for g in b.volumes))
     fnames = list(mesh.faces)
     ptag = {("v", g): i + 1 for i, g in enumerate(groups)}
     ptag.update({("s", f): len(groups) + i + 1 for i, f in enumerate(fnames)})
     lo, hi = mesh.coords.min(axis=0), mesh.coords.max(axis=0)
-    bbox = " ".join(f"{v:.17g}" for v in list(lo) + list(hi))
     vol_ents, surf_ents = [], []
     for g in groups:
         for b in mesh.element_blocks():
-            vol_ents.append((len(vol_ents) + 1, ptag[("v", g)], b.etype, b.conn[b.volumes[g]]))
+            if g in b.volumes:
+                vol_ents.append((len(vol_ents) + 1, ptag[("v", g)], b.etype, _gmsh_conn(b.etype, b.conn[b.volumes[g]])))
     for f in fnames:
         for fs in mesh.face_sets(f):
-            surf_ents.append((len(surf_ents) + 1, ptag[("s", f)], fs.etype, fs.conn))
-    out = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(ptag))]
-    out += [f'3 {ptag[("v", g)]} "{g}"' for g in groups] + [f'2 {ptag[("s", f)]} "{f}"' for f in fnames]
-    out += ["$EndPhysicalNames", "$Entities", f"0 0 {len(surf_ents)} {len(vol_ents)}"]
-    out += [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in surf_ents] + [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in vol_ents]
-    N = mesh.N
-    out += ["$EndEntities", "$Nodes", f"1 {N} 1 {N}", f"3 1 0 {N}"]
-    out += [str(i + 1) for i in range(N)] + [" ".join(f"{v:.17g}" for v in x) for x in mesh.coords]
+            surf_ents.append((len(surf_ents) + 1, ptag[("s", f)], fs.etype, _gmsh_conn(fs.etype, fs.conn)))
     ents = [(2, t, et, c) for t, _, et, c in surf_ents] + [(3, t, et, c) for t, _, et, c in vol_ents]
     ne = sum(c.shape[0] for *_, c in ents)
-    out += ["$EndNodes", "$Elements", f"{len(ents)} {ne} 1 {ne}"]
+    N = mesh.N
+    names = [f'3 {ptag[("v", g)]} "{g}"' for g in groups] + [f'2 {ptag[("s", f)]} "{f}"' for f in fnames]
+    if not binary:
+        bbox = " ".join(f"{v:.17g}" for v in list(lo) + list(hi))
+        out = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(ptag))] + names
+        out += ["$EndPhysicalNames", "$Entities", f"0 0 {len(surf_ents)} {len(vol_ents)}"]
+        out += [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in surf_ents] + [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in vol_ents]
+        out += ["$EndEntities", "$Nodes", f"1 {N} 1 {N}", f"3 1 0 {N}"]
+        out += [str(i + 1) for i in range(N)] + [" ".join(f"{v:.17g}" for v in x) for x in mesh.coords]
+        out += ["$EndNodes", "$Elements", f"{len(ents)} {ne} 1 {ne}"]
+        tag = 1
+        for dim, t, et, c in ents:
+            out.append(f"{dim} {t} {et} {c.shape[0]}")
+            for row in c:
+                out.append(f"{tag} " + " ".join(str(v + 1) for v in row))
+                tag += 1
+        out.append("$EndElements")
+        with open(path, "w") as fh:
+            fh.write("\n".join(out) + "\n")
+        return
+    import struct
+    q = lambda *v: struct.pack(f"<{len(v)}Q", *v)
+    i4 = lambda *v: struct.pack(f"<{len(v)}i", *v)
+    d8 = lambda *v: struct.pack(f"<{len(v)}d", *v)
+    parts = [b"$MeshFormat\n4.1 1 8\n", i4(1), b"\n$EndMeshFormat\n$PhysicalNames\n",
+             ("\n".join([str(len(ptag))] + names) + "\n").encode(), b"$EndPhysicalNames\n$Entities\n",
+             q(0, 0, len(surf_ents), len(vol_ents))]
+    for t, p, _, _ in surf_ents + vol_ents:
+        parts += [i4(t), d8(*lo, *hi), q(1), i4(p), q(0)]
+    parts += [b"\n$EndEntities\n$Nodes\n", q(1, N, 1, N), i4(3, 1, 0), q(N),
+              np.arange(1, N + 1, dtype="<u8").tobytes(), np.ascontiguousarray(mesh.coords, dtype="<f8").tobytes(),
+              b"\n$EndNodes\n$Elements\n", q(len(ents), ne, 1, ne)]
     tag = 1
     for dim, t, et, c in ents:
-        out.append(f"{dim} {t} {et} {c.shape[0]}")
-        for row in c:
-            out.append(f"{tag} " + " ".join(str(v + 1) for v in row))
-            tag += 1
-    out.append("$EndElements")
-    with open(path, "w") as fh:
-        fh.write("\n".join(out) + "\n")
+        rows = np.empty((c.shape[0], c.shape[1] + 1), dtype="<u8")
+        rows[:, 0] = np.arange(tag, tag + c.shape[0])
+        rows[:, 1:] = c + 1
+        tag += c.shape[0]
+        parts += [i4(dim, t, et), q(c.shape[0]), rows.tobytes()]
+    parts.append(b"\n$EndElements\n")
+    with open(path, "wb") as fh:
+        fh.write(b"".join(parts))

@@ -93,3 +93,94 @@ def test_mixed_volume_types_round_trip(tmp_path):
     assert [fs.etype for fs in front] == [3, 2]
     assert np.array_equal(np.unique(r.nodes_on("front")), np.unique(m.nodes_on("front")))
     assert np.array_equal(r.nodes_on("steel"), m.nodes_on("steel"))
+
+
+def _same_mesh(a, b):
+    assert np.array_equal(a.coords, b.coords)
+    for x, y in zip(a.element_blocks(), b.element_blocks()):
+        assert x.etype == y.etype and np.array_equal(x.conn, y.conn)
+        assert sorted(x.volumes) == sorted(y.volumes)
+        for g in x.volumes:
+            assert np.array_equal(x.volumes[g], y.volumes[g])
+    assert sorted(a.faces) == sorted(b.faces)
+    for f in a.faces:
+        for p, q in zip(a.face_sets(f), b.face_sets(f)):
+            assert p.etype == q.etype and np.array_equal(p.conn, q.conn)
+
+
+def test_binary_matches_ascii(tmp_path):
+    """The same mesh written as MSH 4.1 ASCII and binary (little-endian, data
+    size 8) reads back identically, both equal to the written mesh."""
+    from mixed_mesh import hex_prism_box, write_msh
+    m = hex_prism_box()
+    write_msh(m, tmp_path / "a.msh")
+    write_msh(m, tmp_path / "b.msh", binary=True)
+    a, b = read_msh(tmp_path / "a.msh"), read_msh(tmp_path / "b.msh")
+    _same_mesh(a, b)
+    assert [x.etype for x in b.element_blocks()] == [5, 6]
+    assert np.array_equal(b.coords, m.coords)
+
+
+def test_binary_reader_errors(tmp_path):
+    from mixed_mesh import hex_prism_box, write_msh
+    m = hex_prism_box(cells=(2, 2, 2))
+    write_msh(m, tmp_path / "b.msh", binary=True)
+    data = (tmp_path / "b.msh").read_bytes()
+    (tmp_path / "d4.msh").write_bytes(data.replace(b"4.1 1 8", b"4.1 1 4", 1))
+    with pytest.raises(NotImplementedError, match="data size 4"):
+        read_msh(tmp_path / "d4.msh")
+    (tmp_path / "cut.msh").write_bytes(data[: len(data) // 2])
+    with pytest.raises(ValueError):
+        read_msh(tmp_path / "cut.msh")
+    (tmp_path / "noend.msh").write_bytes(data.replace(b"$MeshFormat\n4.1 1 8\n\x01\x00\x00\x00",
+                                                      b"$MeshFormat\n4.1 1 8\n\x07\x00\x00\x00", 1))
+    with pytest.raises(ValueError, match="endianness"):
+        read_msh(tmp_path / "noend.msh")
+
+
+def test_gmsh_recursive_order_reproduces_documented_hex27_quad9():
+    """At order 2 the recursive rule (vertices, edges, face and volume
+    interiors as lower-order elements) gives gmsh's documented hex27 and quad9
+    orders, which the element-local order follows."""
+    from lowlevelfem import refelem as RE
+    from lowlevelfem.msh import gmsh_hex_lattice, gmsh_quad_lattice, gmsh_to_local
+    assert np.array_equal(np.array(gmsh_hex_lattice(2)), RE.hex_node_lattice(2))
+    assert np.array_equal(np.array(gmsh_quad_lattice(2)), RE.quad_node_lattice(2))
+    for et, npe in ((92, 64), (93, 125), (36, 16), (37, 25)):
+        perm = gmsh_to_local(et)
+        assert np.array_equal(np.sort(perm), np.arange(npe))
+    # vertices and edge nodes keep their places; only face and volume interiors move
+    assert np.array_equal(gmsh_to_local(93)[:8 + 12 * 3], np.arange(8 + 12 * 3))
+    assert not np.array_equal(gmsh_to_local(93), np.arange(125))
+
+
+@pytest.mark.parametrize("order", [3, 4])
+@pytest.mark.parametrize("binary", [False, True], ids=["ascii", "binary"])
+def test_high_order_hex_round_trip(tmp_path, order, binary):
+    """hex64 / hex125 meshes (quad16 / quad25 faces) written in gmsh's node
+    order read back in the element-local order: the same connectivity, the
+    same faces, and the same stiffness from the oracle.  In the written file,
+    every element's gmsh-ordered nodes sit at the gmsh lattice positions of
+    its own corner box (structured, unperturbed mesh)."""
+    from mixed_mesh import write_msh
+    from lowlevelfem.msh import gmsh_hex_lattice
+    from lowlevelfem import refelem as RE
+    from conftest import hooke_symgrad_D
+    m = M.structured_box_mesh(cells=(2, 1, 2), lx=1.0, ly=0.5, lz=1.0, order=order)
+    m.volumes["body"] = np.arange(m.E)
+    path = tmp_path / f"hex{order}.msh"
+    write_msh(m, path, binary=binary)
+    r = read_msh(path)
+    _same_mesh(r, m)
+    # gmsh positions: node j of an element in gmsh order at lo + lattice_j / p * (hi - lo)
+    from lowlevelfem.msh import gmsh_to_local
+    gconn = m.conn[:, np.argsort(gmsh_to_local(m.etype))]
+    lat = np.array(gmsh_hex_lattice(order), dtype=np.float64) / order
+    for e in range(m.E):
+        X = m.coords[gconn[e]]
+        lo, hi = X.min(axis=0), X.max(axis=0)
+        assert np.allclose(X, lo + lat * (hi - lo), atol=1e-14)
+    tab = RE.rule_tables(m.etype, 2 * order + 1)
+    cp, rv = O.csc_pattern(m)
+    D = hooke_symgrad_D()
+    assert np.array_equal(O.stiffness_csc(r, tab, D, *O.csc_pattern(r)), O.stiffness_csc(m, tab, D, cp, rv))
