@@ -176,7 +176,7 @@ struct llfem_mesh {
   int64_t nmulti = 0;             // CSC blocks with more than one contribution
   int32_t* d_mptr = nullptr;      // [nmulti+1] their staged blocks d_mptr[m] .. d_mptr[m+1]
   int64_t* d_mref = nullptr;      // [nmulti][2] value offset 9 a0 + 3 s and panel stride 3 R of each
-  void* d_ho = nullptr;           // staged blocks + Z
+  void* d_ho = nullptr;           // staged blocks, general material, gradient table by position
   size_t ho_bytes = 0;
   // AMG hierarchies kept per support (key: hash of the constrained-DoF list),
   // each with the device mask it was built for
@@ -260,7 +260,8 @@ class BatchSolver {
   bool any_running() const;
 
  private:
-  int update(SolveSys* s, bool init);
+  int update_many(const std::vector<SolveSys*>& ss, bool init);
+  int host_slots(size_t n);
   int rebuild_tables();
   void release(SolveSys* s);
   int launch_chunk(int iters);       // the iterations of one chunk, as kernel launches
@@ -279,6 +280,8 @@ class BatchSolver {
   int32_t* d_sb_ = nullptr;
   int32_t* d_vb_ = nullptr;
   double* d_states_ = nullptr;
+  double* hslot_ = nullptr;          // pinned readback slots of the updates
+  size_t hslot_n_ = 0;
   int nsys_ = 0;
   int64_t nsb_ = 0, nvb_ = 0;
 };

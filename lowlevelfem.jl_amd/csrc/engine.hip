@@ -310,6 +310,9 @@ __global__ __launch_bounds__(kSB) void k_ru_resid(int64_t n, const double* __res
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// scal[SC_STATE] = v (one thread; an eager write that needs no host copy)
+__global__ void k_set_state(double* __restrict__ scal, double v) { scal[SC_STATE] = v; }
+
 __global__ __launch_bounds__(kSB) void k_sum(int64_t n, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double sh[kSB];
   double v = 0.0;
@@ -529,6 +532,7 @@ BatchSolver::BatchSolver(llfem_ctx* c, bool f64_vectors) : ctx_(c), f64v_(f64_ve
 
 BatchSolver::~BatchSolver() {
   drop_graph();
+  if (hslot_) (void)hipHostFree(hslot_);
   for (SolveSys* s : sys_) release(s);
   sys_.clear();
   if (tables_) dfree(ctx_, tables_, tables_bytes_);
@@ -550,14 +554,37 @@ size_t BatchSolver::bytes_needed(int64_t N, int64_t nadj, bool f64_vectors) {
          sizeof(float) * (9 * (size_t)N + 9 * (size_t)nadj) + 16 * 256;   // 14 buffers, 256-B aligned
 }
 
-// Reliable update of one system: fold x32 into xh, recompute the FP64
-// residual, test convergence, restart the FP32 recursion.
-int BatchSolver::update(SolveSys* s, bool init) {
+// Pinned host slots for the update readbacks (grown on demand).
+int BatchSolver::host_slots(size_t n) {
+  if (hslot_n_ >= n) return LLFEM_OK;
+  if (hslot_) (void)hipHostFree(hslot_);
+  hslot_ = nullptr;
+  hslot_n_ = 0;
+  LLFEM_HIP(hipHostMalloc((void**)&hslot_, sizeof(double) * n, hipHostMallocDefault));
+  hslot_n_ = n;
+  return LLFEM_OK;
+}
+
+// Reliable updates of several systems: fold x32 into xh, recompute the FP64
+// residual, test convergence, restart the FP32 recursion.  The host waits a
+// fixed number of times per call, whatever the number of systems:
+//   1. every system's trial solution and true residual, one readback;
+//   2. the decisions (engine_policy.h); rolled-back systems get the residual
+//      of their kept solution, one more readback if there are any;
+//   3. stop, or restart the recursion; AMG systems that keep a rescaled z
+//      read r.z and the drift back, one readback for all of them;
+//   4. those whose rescaled z fails the test get a V-cycle.
+// Each system's launches are the same as for an update of that system alone.
+int BatchSolver::update_many(const std::vector<SolveSys*>& ss, bool init) {
+  if (ss.empty()) return LLFEM_OK;
   hipStream_t st = ctx_->stream;
-  const SysSpec& sp = s->spec;
-  const int64_t n = s->n;
-  // r = b - K x in FP64 for x (NULL: x = 0); r2 = ||r||^2 read back
-  auto residual = [&](const double* x, double* r2) -> int {
+  const size_t m = ss.size();
+  LLFEM_TRY(host_slots(4 * m));   // per system: r^2, iterations, r.z, drift^2
+  double* h = hslot_;
+  // r = b - K x in FP64 for x (NULL: x = 0); ||r||^2 into scal[SC_N - 1]
+  auto residual = [&](SolveSys* s, const double* x) {
+    const SysSpec& sp = s->spec;
+    const int64_t n = s->n;
     if (x) {
       k_spmv64<<<blocks(sp.N * kSG), kSB, 0, st>>>(sp.N, sp.adj_ptr, sp.adj, sp.val64, x, sp.fixed, s->y);
       k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, s->y, sp.fixed, s->rh, s->part);
@@ -565,94 +592,120 @@ int BatchSolver::update(SolveSys* s, bool init) {
       k_ru_resid<<<blocks(n), kSB, 0, st>>>(n, s->b, nullptr, sp.fixed, s->rh, s->part);
     }
     k_sum<<<1, kSB, 0, st>>>(blocks(n), s->part, s->scal + SC_N - 1);
-    LLFEM_HIP(hipMemcpyAsync(r2, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
-    LLFEM_HIP(hipStreamSynchronize(st));
-    return LLFEM_OK;
   };
-  double r2 = 0.0, it = 0.0;
-  bool fresh = init;   // restart the Krylov recursion from scratch (p = z)
+  // 1. trial solutions xt = xh + x / scale and their residuals
+  for (size_t i = 0; i < m; ++i) {
+    SolveSys* s = ss[i];
+    const int64_t n = s->n;
+    if (!init) {
+      if (f64v_) k_ru_trial<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh, s->xt);
+      else k_ru_trial<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh, s->xt);
+      residual(s, s->xt);
+    } else {
+      residual(s, nullptr);
+    }
+    LLFEM_HIP(hipMemcpyAsync(h + 4 * i, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    LLFEM_HIP(hipMemcpyAsync(h + 4 * i + 1, s->scal + SC_ITER, sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  LLFEM_HIP(hipStreamSynchronize(st));
+  // 2. adoption or rollback.  LLFEM_ENGINE_FORCE_ROLLBACK=k treats the k-th
+  // update's trial as not finite (a host-side switch for testing the
+  // rollback path).
+  std::vector<uint8_t> fresh(m, init ? 1 : 0);   // restart the Krylov recursion from scratch (p = z)
   if (!init) {
-    if (f64v_) k_ru_trial<double><<<blocks(n), kSB, 0, st>>>(n, (double*)s->x, 1.0 / s->scale, s->xh, s->xt);
-    else k_ru_trial<float><<<blocks(n), kSB, 0, st>>>(n, (float*)s->x, 1.0 / s->scale, s->xh, s->xt);
-    LLFEM_TRY(residual(s->xt, &r2));
-    // LLFEM_ENGINE_FORCE_ROLLBACK=k treats the k-th update's trial as not
-    // finite (a host-side switch for testing the rollback path)
     const char* force_env = std::getenv("LLFEM_ENGINE_FORCE_ROLLBACK");
     const int64_t force_rb = force_env ? (int64_t)std::atoll(force_env) : 0;
-    if (force_rb > 0 && s->book.updates + 1 == force_rb && s->book.rollbacks == 0) r2 = NAN;
-    if (adopt_trial(r2, &s->book)) {
-      std::swap(s->xh, s->xt);
-    } else {
-      // rollback: keep xh, restart the recursion from it
-      fresh = true;
-      LLFEM_TRY(residual(s->xh, &r2));
+    bool any = false;
+    for (size_t i = 0; i < m; ++i) {
+      SolveSys* s = ss[i];
+      double r2 = h[4 * i];
+      if (force_rb > 0 && s->book.updates + 1 == force_rb && s->book.rollbacks == 0) r2 = NAN;
+      if (adopt_trial(r2, &s->book)) {
+        std::swap(s->xh, s->xt);
+      } else {
+        // rollback: keep xh, restart the recursion from it
+        fresh[i] = 1;
+        any = true;
+        residual(s, s->xh);
+        LLFEM_HIP(hipMemcpyAsync(h + 4 * i, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+      }
     }
-  } else {
-    LLFEM_TRY(residual(nullptr, &r2));
+    if (any) LLFEM_HIP(hipStreamSynchronize(st));
   }
-  LLFEM_HIP(hipMemcpyAsync(&it, s->scal + SC_ITER, sizeof(double), hipMemcpyDeviceToHost, st));
-  LLFEM_HIP(hipStreamSynchronize(st));
-  const double rn = std::sqrt(r2);
-  s->iters = (int64_t)it;
-  if (init) s->bnorm = rn;
-  else ++s->book.updates;
-  const int status = update_status(rn, s->bnorm, sp.reltol, s->iters, sp.maxiter, s->book);
-  if (status != kRunning) {
-    s->status = status;
-    if (status != kConverged) s->relres = s->bnorm > 0 && std::isfinite(rn) ? rn / s->bnorm : 0.0;
-    const double idle = ST_IDLE;
-    LLFEM_HIP(hipMemcpyAsync(s->scal + SC_STATE, &idle, sizeof(double), hipMemcpyHostToDevice, st));
-    LLFEM_HIP(hipStreamSynchronize(st));
-    return LLFEM_OK;
-  }
-  const double s_new = 1.0 / rn;
-  const double pscale = s_new / s->scale;
-  s->scale = s_new;
-  const double tol = sp.reltol * s->bnorm * s_new;
-  if (s->amg) {
-    const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
-    double* vpart = s->part + nsblk;
-    if (fresh) {   // z = M r by a V-cycle of this system alone, p = z
-      k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, 1, (float*)s->r,
-                                                        (float*)s->p, vpart);
-      LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
-      LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    } else {
-      k_ru_rescale_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, (float)pscale, (float*)s->r,
-                                                        (float*)s->z, (float*)s->p, vpart);
+  // 3. stop or restart
+  std::vector<uint8_t> check(m, 0);   // AMG systems whose rescaled z is tested
+  std::vector<double> tol2(m, 0.0), s_new(m, 0.0), pscale(m, 0.0);
+  bool any_check = false;
+  for (size_t i = 0; i < m; ++i) {
+    SolveSys* s = ss[i];
+    const SysSpec& sp = s->spec;
+    const int64_t n = s->n;
+    const double rn = std::sqrt(h[4 * i]);
+    s->iters = (int64_t)h[4 * i + 1];
+    if (init) s->bnorm = rn;
+    else ++s->book.updates;
+    const int status = update_status(rn, s->bnorm, sp.reltol, s->iters, sp.maxiter, s->book);
+    if (status != kRunning) {
+      s->status = status;
+      if (status != kConverged) s->relres = s->bnorm > 0 && std::isfinite(rn) ? rn / s->bnorm : 0.0;
+      k_set_state<<<1, 1, 0, st>>>(s->scal, (double)ST_IDLE);
+      continue;
     }
-    k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
-    if (!fresh) {
-      // The rescaled z is consistent only if r.z > 0 and the true residual is
-      // close to the rescaled recursive one (relative drift below 1/4);
-      // otherwise recompute z = M r and restart the direction from it.
-      k_sum<<<1, kSB, 0, st>>>(nvblk, vpart + 2 * nvblk, s->scal + SC_N - 1);
-      double h[2] = {0.0, 0.0};   // r.z, drift^2
-      LLFEM_HIP(hipMemcpyAsync(&h[0], s->scal + SC_RHO, sizeof(double), hipMemcpyDeviceToHost, st));
-      LLFEM_HIP(hipMemcpyAsync(&h[1], s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
-      LLFEM_HIP(hipStreamSynchronize(st));
-      // r is the true residual scaled to unit norm, so rr = 1 up to rounding
-      if (!amg_keep_rescaled(h[0], h[1], kDriftAmg)) {
-        ++s->refreshes;
-        k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new, pscale, 1, (float*)s->r,
+    s_new[i] = 1.0 / rn;
+    pscale[i] = s_new[i] / s->scale;
+    s->scale = s_new[i];
+    const double tol = sp.reltol * s->bnorm * s_new[i];
+    tol2[i] = tol * tol;
+    if (s->amg) {
+      const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
+      double* vpart = s->part + nsblk;
+      if (fresh[i]) {   // z = M r by a V-cycle of this system alone, p = z
+        k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new[i], pscale[i], 1, (float*)s->r,
                                                           (float*)s->p, vpart);
         LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
         LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-        k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol * tol, kDeltaAmg * kDeltaAmg,
-                                         (double)kMaxRunAmg);
+      } else {
+        k_ru_rescale_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new[i], (float)pscale[i],
+                                                          (float*)s->r, (float*)s->z, (float*)s->p, vpart);
       }
+      k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol2[i], kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
+      if (!fresh[i]) {
+        // the rescaled z is consistent only if r.z > 0 and the true residual
+        // is close to the rescaled recursive one (amg_keep_rescaled)
+        k_sum<<<1, kSB, 0, st>>>(nvblk, vpart + 2 * nvblk, s->scal + SC_N - 1);
+        LLFEM_HIP(hipMemcpyAsync(h + 4 * i + 2, s->scal + SC_RHO, sizeof(double), hipMemcpyDeviceToHost, st));
+        LLFEM_HIP(hipMemcpyAsync(h + 4 * i + 3, s->scal + SC_N - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        check[i] = 1;
+        any_check = true;
+      }
+      continue;
     }
-    LLFEM_HIP(hipGetLastError());
-    return LLFEM_OK;
+    const int64_t nb = blocks(sp.N);
+    if (f64v_)
+      k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new[i], pscale[i], fresh[i] ? 1 : 0, s->binv,
+                                                         (double*)s->r, (double*)s->z, (double*)s->p, s->part);
+    else
+      k_ru_restart<float><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new[i], pscale[i], fresh[i] ? 1 : 0, s->binv,
+                                                        (float*)s->r, (float*)s->z, (float*)s->p, s->part);
+    k_ru_scal<<<1, kSB, 0, st>>>(nb, s->part, s->scal, tol2[i], kDelta * kDelta);
   }
-  const int64_t nb = blocks(sp.N);
-  if (f64v_)
-    k_ru_restart<double><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, fresh ? 1 : 0, s->binv,
-                                                       (double*)s->r, (double*)s->z, (double*)s->p, s->part);
-  else
-    k_ru_restart<float><<<(unsigned)nb, kSB, 0, st>>>(sp.N, s->rh, s_new, pscale, fresh ? 1 : 0, s->binv,
-                                                      (float*)s->r, (float*)s->z, (float*)s->p, s->part);
-  k_ru_scal<<<1, kSB, 0, st>>>(nb, s->part, s->scal, tol * tol, kDelta * kDelta);
+  // 4. AMG systems whose rescaled z failed: z = M r, restart the direction from it
+  if (any_check) {
+    LLFEM_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < m; ++i) {
+      if (!check[i] || amg_keep_rescaled(h[4 * i + 2], h[4 * i + 3], kDriftAmg)) continue;
+      SolveSys* s = ss[i];
+      const SysSpec& sp = s->spec;
+      const int32_t nsblk = (int32_t)spmv_blocks(sp.N), nvblk = (int32_t)vec_blocks(sp.N);
+      double* vpart = s->part + nsblk;
+      ++s->refreshes;
+      k_ru_restart_amg<<<(unsigned)nvblk, kSB, 0, st>>>(sp.N, nvblk, s->rh, s_new[i], pscale[i], 1, (float*)s->r,
+                                                        (float*)s->p, vpart);
+      LLFEM_TRY(amg_apply_solo(ctx_, s->amg));
+      LLFEM_HIP(hipMemcpyAsync(s->p, s->z, sizeof(float) * s->n, hipMemcpyDeviceToDevice, st));
+      k_ru_scal_amg<<<1, kSB, 0, st>>>(nvblk, vpart, s->scal, tol2[i], kDeltaAmg * kDeltaAmg, (double)kMaxRunAmg);
+    }
+  }
   LLFEM_HIP(hipGetLastError());
   return LLFEM_OK;
 }
@@ -718,7 +771,7 @@ int BatchSolver::add(const SysSpec& spec, SolveSys** out) {
       return rc;
     }
   }
-  rc = update(s, true);
+  rc = update_many({s}, true);
   if (rc != LLFEM_OK) {
     release(s);
     return rc;
@@ -847,18 +900,19 @@ int BatchSolver::run(int iters, std::vector<SolveSys*>* finished) {
   LLFEM_HIP(hipMemcpyAsync(h.data(), d_states_, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
   LLFEM_HIP(hipStreamSynchronize(st));
   LLFEM_HIP(hipGetLastError());
+  std::vector<SolveSys*> paused;
   for (int i = 0; i < nsys_; ++i) {
     SolveSys* s = live_[i];
     s->iters = (int64_t)h[2 * i + 1];
     const int state = (int)h[2 * i];
-    if (state == ST_PENDING || state == ST_PAUSED) {
-      LLFEM_TRY(update(s, false));
-      if (s->status != 0) {
-        dirty_ = true;
-        if (finished) finished->push_back(s);
-      }
-    }
+    if (state == ST_PENDING || state == ST_PAUSED) paused.push_back(s);
   }
+  LLFEM_TRY(update_many(paused, false));
+  for (SolveSys* s : paused)
+    if (s->status != kRunning) {
+      dirty_ = true;
+      if (finished) finished->push_back(s);
+    }
   return LLFEM_OK;
 }
 
