@@ -265,7 +265,7 @@ int geometry_state(llfem_mesh* m, const llfem_op_desc* d, bool* affine_out) {
   return LLFEM_OK;
 }
 
-int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const double* d_scale) {
+int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const double* d_scale, bool f32) {
   llfem_ctx* ctx = m->ctx;
   const int npe = m->npe, q = d->q;
   const size_t geo_doubles = (size_t)kGeoStride * m->E * (affine ? 1 : q);
@@ -276,8 +276,10 @@ int run_geometry(llfem_mesh* m, const llfem_op_desc* d, bool affine, const doubl
     const double* dN0 = m->d_tab;   // gradients at the first point
 #define LLFEM_GEO_AFFINE(n)                                                                        \
   case n:                                                                                          \
-    k_geometry_affine<n><<<g, 256, 0, ctx->stream>>>(m->E, m->N, m->d_xyz, m->d_conn, dN0, m->d_geo, \
-                                                      d_scale);                                    \
+    if (f32)                                                                                       \
+      k_geometry_affine<n, true><<<g, 256, 0, ctx->stream>>>(m->E, m->d_xyz4, m->d_conn, dN0, m->d_geo, d_scale); \
+    else                                                                                           \
+      k_geometry_affine<n, false><<<g, 256, 0, ctx->stream>>>(m->E, m->d_xyz4, m->d_conn, dN0, m->d_geo, d_scale); \
     return LLFEM_OK;
     switch (npe) {
       LLFEM_GEO_AFFINE(4)
@@ -378,7 +380,7 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     LLFEM_TRY(dalloc(ctx, &d_scale, (size_t)m->E));
     LLFEM_HIP(hipMemcpyAsync(d_scale, d->element_scale, sizeof(double) * m->E, hipMemcpyHostToDevice, st));
   }
-  const int grc = run_geometry(m, d, affine, d_scale);
+  const int grc = run_geometry(m, d, affine, d_scale, fp32);
   if (d_scale) {
     LLFEM_HIP(hipStreamSynchronize(st));   // the geometry kernel has read it
     drelease(ctx, d_scale, (size_t)m->E);

@@ -115,8 +115,9 @@ struct llfem_mesh {
   llfem_ctx* ctx = nullptr;
   int64_t N = 0, E = 0;
   int32_t npe = 0, etype = 0, pdim = 3;
-  // coordinates SoA x[N] y[N] z[N]; connectivity SoA [npe][E]
+  // coordinates SoA x[N] y[N] z[N] and node-major (x, y, z, 0); connectivity SoA [npe][E]
   double* d_xyz = nullptr;
+  double4* d_xyz4 = nullptr;
   int32_t* d_conn = nullptr;
   // node adjacency (sorted) = the structural pattern at node level
   int64_t nadj = 0;

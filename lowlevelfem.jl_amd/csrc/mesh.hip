@@ -399,12 +399,17 @@ int build_pattern(llfem_mesh* m) {
 
 namespace {
 
-__global__ void k_soa_coords(int64_t N, const double* __restrict__ xyz, double* __restrict__ soa) {
+// caller's node-major coordinates -> SoA x[N] y[N] z[N] and the padded
+// node-major copy (x, y, z, 0) the affine geometry kernel gathers
+__global__ void k_soa_coords(int64_t N, const double* __restrict__ xyz, double* __restrict__ soa,
+                             double4* __restrict__ xyz4) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
-  soa[j] = xyz[3 * j];
-  soa[N + j] = xyz[3 * j + 1];
-  soa[2 * N + j] = xyz[3 * j + 2];
+  const double x = xyz[3 * j], y = xyz[3 * j + 1], z = xyz[3 * j + 2];
+  soa[j] = x;
+  soa[N + j] = y;
+  soa[2 * N + j] = z;
+  xyz4[j] = make_double4(x, y, z, 0.0);
 }
 
 }  // namespace
@@ -416,6 +421,7 @@ using namespace llfem;
 static void mesh_free(llfem_mesh* m) {
   llfem_ctx* c = m->ctx;
   drelease(c, m->d_xyz, 3 * m->N);
+  drelease(c, m->d_xyz4, m->N);
   drelease(c, m->d_conn, m->E * m->npe);
   drelease(c, m->d_adj_ptr, m->N + 1);
   drelease(c, m->d_adj, m->nadj);
@@ -482,6 +488,7 @@ int llfem_mesh_create(llfem_ctx* ctx, const double* xyz, int64_t N, int32_t elem
   double* d_aos = nullptr;
   do {
     if ((s = dalloc(ctx, &m->d_xyz, 3 * N))) break;
+    if ((s = dalloc(ctx, &m->d_xyz4, N))) break;
     if ((s = dalloc(ctx, &m->d_conn, E * npe))) break;
     if ((s = dalloc(ctx, &d_aos, 3 * N))) break;
     if (hipMemcpyAsync(d_aos, xyz, sizeof(double) * 3 * N, hipMemcpyHostToDevice, ctx->stream) ||
@@ -490,7 +497,7 @@ int llfem_mesh_create(llfem_ctx* ctx, const double* xyz, int64_t N, int32_t elem
       s = fail(LLFEM_EHIP, "llfem_mesh_create: upload failed");
       break;
     }
-    k_soa_coords<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(N, d_aos, m->d_xyz);
+    k_soa_coords<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(N, d_aos, m->d_xyz, m->d_xyz4);
     if ((s = build_pattern(m))) break;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
       s = fail(LLFEM_EHIP, "llfem_mesh_create: device error");

@@ -89,6 +89,8 @@ __global__ void k_geom_check(int64_t E, int64_t N, int npe, int q, const double*
 // (element, point), wd = |det J_p| w_p.
 constexpr int kGeoStride = 16;   // doubles
 
+// F32: also write the FP32 copy that the FP32-integration kernels read
+template <bool F32 = true>
 __device__ __forceinline__ void geometry_store(double* g, const double* Mi, double wd) {
   double2* g2 = reinterpret_cast<double2*>(g);
   g2[0] = make_double2(Mi[0], Mi[1]);
@@ -96,6 +98,7 @@ __device__ __forceinline__ void geometry_store(double* g, const double* Mi, doub
   g2[2] = make_double2(Mi[4], Mi[5]);
   g2[3] = make_double2(Mi[6], Mi[7]);
   g2[4] = make_double2(Mi[8], wd);
+  if constexpr (!F32) return;
   float4* f4 = reinterpret_cast<float4*>(g + 10);
   f4[0] = make_float4((float)Mi[0], (float)Mi[1], (float)Mi[2], (float)Mi[3]);
   f4[1] = make_float4((float)Mi[4], (float)Mi[5], (float)Mi[6], (float)Mi[7]);
@@ -124,9 +127,11 @@ __device__ __forceinline__ void geometry_one(int64_t E, int64_t N, int npe, int 
 // Affine elements with a compile-time node count: the same sums in the same
 // order as jacobian(), with every connectivity and coordinate load issued
 // before the first is used (the run-time loop waits for each node's gathers
-// in turn).
-template <int NPE>
-__global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, int64_t N, const double* __restrict__ xyz,
+// in turn).  Coordinates come from the node-major copy (x, y, z, 0 per node:
+// two 16-byte loads from one line per node instead of three loads from three
+// arrays); F32 selects the FP32 copy in the record.
+template <int NPE, bool F32>
+__global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, const double4* __restrict__ xyz4,
                                                          const int32_t* __restrict__ conn,
                                                          const double* __restrict__ dN, double* __restrict__ geo,
                                                          const double* __restrict__ scale) {
@@ -138,9 +143,10 @@ __global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, int64_t N, c
   double x[NPE], y[NPE], z[NPE];
 #pragma unroll
   for (int a = 0; a < NPE; ++a) {
-    x[a] = xyz[v[a]];
-    y[a] = xyz[N + v[a]];
-    z[a] = xyz[2 * N + v[a]];
+    const double4 c = xyz4[v[a]];
+    x[a] = c.x;
+    y[a] = c.y;
+    z[a] = c.z;
   }
   double J[9], Mi[9];
 #pragma unroll
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(256) void k_geometry_affine(int64_t E, int64_t N, c
     J[6] += dz * g0; J[7] += dz * g1; J[8] += dz * g2;
   }
   const double det = inv3(J, Mi);
-  geometry_store(geo + (size_t)e * kGeoStride, Mi, fabs(det) * (scale ? scale[e] : 1.0));
+  geometry_store<F32>(geo + (size_t)e * kGeoStride, Mi, fabs(det) * (scale ? scale[e] : 1.0));
 }
 
 // dNT[(a * q + p) * 3 + c] = dN[(p * npe + a) * 3 + c]: the gradient table
