@@ -106,13 +106,14 @@ def cpu_baseline(cells):
 
 
 PROFILE = os.path.join(ROOT, "profiles", "panel_assembly_tet10_beam_40x40x160.md")
+PROFILE_HEX125 = os.path.join(ROOT, "profiles", "gram_assembly_hex125_12cube.md")
 
 
-def measured_traffic():
+def measured_traffic(profile=PROFILE):
     """HBM bytes per assembly from the committed rocprofv3 FETCH/WRITE summary
-    of this workload (null when the summary is absent)."""
+    of a workload (null when the summary is absent)."""
     try:
-        for line in open(PROFILE):
+        for line in open(profile):
             if "traffic_bytes_per_assembly:" in line:
                 return float(line.split(":")[1])
     except OSError:
@@ -312,7 +313,7 @@ def bench_hex125(args):
                                "5x5x5 Gauss, FP64 stiffness into CSC",
                    "elements": m.E, "nodes": m.N, "dofs": 3 * m.N, "nnz": nnz},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(PROFILE_HEX125)},
         "flops_per_element": {"reference_dense_btdb": f_ref, "gram_useful": f_gram,
                               "gram_mfma_issued": f_exec},
         "kernel_ms": {"assembly_events": k64, "gram_kernel_events": g64},
