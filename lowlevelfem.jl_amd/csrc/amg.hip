@@ -491,6 +491,23 @@ struct HostLevel {
   HVec<int32_t> gblk, gcol;
 };
 
+// Run f(lo, hi) over [0, n) split across host threads.
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), 16));
+  if (n < 4096 || nt == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t step = (n + nt - 1) / nt;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t lo = t * step, hi = std::min(n, lo + step);
+    if (lo < hi) th.emplace_back([&f, lo, hi]() { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
+}
+
 // A graph over real nodes: the neighbours of i are idx[e], e in
 // [ptr[i], ptr[i+1]), except i itself and, with `flag`, entries flagged 0.
 struct Graph {
@@ -538,18 +555,28 @@ int64_t aggregate(int64_t n, const Graph& S, const Graph& F, std::vector<int32_t
   for (int64_t i = 0; i < n; ++i) {   // phase 1: root + all strong neighbours, all unaggregated
     if (agg[i] >= 0) continue;
     bool ok = true, any = false;
-    S.each(i, [&](int32_t j) {
+    for (int64_t e = S.ptr[i]; e < S.ptr[i + 1]; ++e) {   // stops at the first aggregated one
+      const int32_t j = S.idx[e];
+      if (j == i || (S.flag && !S.flag[e])) continue;
       any = true;
-      ok = ok && agg[j] < 0;
-    });
+      if (agg[j] >= 0) {
+        ok = false;
+        break;
+      }
+    }
     if (!ok || !any) continue;
     agg[i] = (int32_t)na;
     S.each(i, [&](int32_t j) { agg[j] = (int32_t)na; });
     ++na;
   }
+  // phase 2: join the phase-1 aggregate coupled most strongly.  It reads the
+  // phase-1 snapshot only, so the nodes are independent (host threads).
   std::vector<int32_t> snap = agg, buf;
-  for (int64_t i = 0; i < n; ++i)     // phase 2: join the phase-1 aggregate coupled most strongly
-    if (agg[i] < 0) agg[i] = most_coupled(S, i, snap, buf);
+  parallel_for(n, [&](int64_t lo, int64_t hi) {
+    std::vector<int32_t> b;
+    for (int64_t i = lo; i < hi; ++i)
+      if (snap[i] < 0) agg[i] = most_coupled(S, i, snap, b);
+  });
   for (int64_t i = 0; i < n; ++i) {   // phase 3a: leftovers with unaggregated strong neighbours
     if (agg[i] >= 0) continue;
     bool any = false;
@@ -575,22 +602,6 @@ int64_t aggregate(int64_t n, const Graph& S, const Graph& F, std::vector<int32_t
   return na;
 }
 
-// Run f(lo, hi) over [0, n) split across host threads.
-template <class F>
-void parallel_for(int64_t n, F&& f) {
-  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), 16));
-  if (n < 4096 || nt == 1) {
-    f((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  const int64_t step = (n + nt - 1) / nt;
-  for (int64_t t = 0; t < nt; ++t) {
-    const int64_t lo = t * step, hi = std::min(n, lo + step);
-    if (lo < hi) th.emplace_back([&f, lo, hi]() { f(lo, hi); });
-  }
-  for (auto& x : th) x.join();
-}
 
 // Modified Gram-Schmidt (two passes) of a rows x 6 matrix (row-major) in
 // place; R (6x6, row-major) with B = Q R; dropped columns are zeroed in Q and
@@ -679,8 +690,19 @@ template <class NullRow>
 void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, int64_t n_real, NullRow&& nullrow,
              const Graph& S, const Graph& F, HostLevel& C, std::vector<double>& Bc, std::vector<int64_t>& nbp,
              std::vector<int32_t>& nbi) {
+  using clk = std::chrono::steady_clock;
+  const bool trace = std::getenv("LLFEM_AMG_TRACE") != nullptr;
+  auto tick = [&](const char* what, clk::time_point& t) {
+    if (!trace) return;
+    const auto now = clk::now();
+    std::fprintf(stderr, "[amg]   N %lld %s %.1f ms\n", (long long)L.N, what,
+                 std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  };
+  auto tp = clk::now();
   std::vector<int32_t> ragg;
   const int64_t na = aggregate(n_real, S, F, ragg);
+  tick("aggregate", tp);
   L.nagg = na;
   L.agg.assign((size_t)L.N, -1);
   for (int64_t a = 0; a < n_real; ++a)
@@ -696,6 +718,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
     for (int64_t j = 0; j < L.N; ++j)
       if (L.agg[j] >= 0) L.mem[fill[L.agg[j]]++] = (int32_t)j;
   }
+  tick("members", tp);
   // tentative prolongator and coarse near-null space (aggregates in parallel)
   L.Q.resize((size_t)L.N * 18);
   parallel_for(L.N, [&](int64_t lo, int64_t hi) { std::fill(L.Q.begin() + 18 * lo, L.Q.begin() + 18 * hi, 0.f); });
@@ -722,6 +745,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
       for (int u = 0; u < 6; ++u) C.mask[6 * a + u] = dropped[u] ? 1 : 0;
     }
   });
+  tick("prolongator", tp);
   // per aggregate: neighbour aggregates (every fine block couples), then the
   // Galerkin contribution lists of its pairs, fine blocks in ascending order.
   // Two passes over the aggregates in parallel (sizes, then contents at
@@ -800,28 +824,32 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
       }
     }
   });
+  tick("galerkin lists", tp);
   // coarse pseudo-node pattern: 2A and 2A+1 both list [2B, 2B+1 for B in nb(A)]
   const int64_t npairs = nbp[na];
   C.adj_ptr.assign((size_t)C.N + 1, 0);
   C.adj.assign((size_t)4 * npairs, 0);
   C.dslot.assign((size_t)C.N, 0);
   L.pair_o.assign((size_t)npairs * 2, 0);
-  for (int64_t a = 0; a < na; ++a) {
-    const int64_t r = 2 * (nbp[a + 1] - nbp[a]);
-    C.adj_ptr[2 * a] = 4 * nbp[a];
-    C.adj_ptr[2 * a + 1] = 4 * nbp[a] + r;
-    C.adj_ptr[2 * a + 2] = 4 * nbp[a + 1];
-    for (int s = 0; s < 2; ++s) {
-      const int64_t o = C.adj_ptr[2 * a + s];
-      for (int64_t k = 0; k < nbp[a + 1] - nbp[a]; ++k) {
-        const int32_t B = nbi[nbp[a] + k];
-        C.adj[o + 2 * k] = 2 * B;
-        C.adj[o + 2 * k + 1] = 2 * B + 1;
-        if (B == a) C.dslot[2 * a + s] = (int32_t)(2 * k + s);
-        L.pair_o[2 * (nbp[a] + k) + s] = (int32_t)(o + 2 * k);
+  parallel_for(na, [&](int64_t lo, int64_t hi) {   // each aggregate writes its own ranges
+    for (int64_t a = lo; a < hi; ++a) {
+      const int64_t r = 2 * (nbp[a + 1] - nbp[a]);
+      C.adj_ptr[2 * a] = 4 * nbp[a];
+      C.adj_ptr[2 * a + 1] = 4 * nbp[a] + r;
+      for (int s = 0; s < 2; ++s) {
+        const int64_t o = 4 * nbp[a] + s * r;
+        for (int64_t k = 0; k < nbp[a + 1] - nbp[a]; ++k) {
+          const int32_t B = nbi[nbp[a] + k];
+          C.adj[o + 2 * k] = 2 * B;
+          C.adj[o + 2 * k + 1] = 2 * B + 1;
+          if (B == a) C.dslot[2 * a + s] = (int32_t)(2 * k + s);
+          L.pair_o[2 * (nbp[a] + k) + s] = (int32_t)(o + 2 * k);
+        }
       }
     }
-  }
+  });
+  C.adj_ptr[2 * na] = 4 * nbp[na];
+  tick("coarse pattern", tp);
 }
 
 }  // namespace
@@ -935,6 +963,7 @@ int amg_build_host(const AmgInput& in, AmgHost* H) {
     nbp_prev.swap(nbp);
     nbi_prev.swap(nbi);
   }
+  const auto tc = std::chrono::steady_clock::now();
   // every index the kernels follow stays inside its array
   for (size_t l = 0; l + 1 < H->lev.size(); ++l) {
     const HostLevel& L = H->lev[l];
@@ -955,6 +984,9 @@ int amg_build_host(const AmgInput& in, AmgHost* H) {
   }
   if (3 * H->lev.back().N > kMaxDense) return fail(LLFEM_EUNSUP, "AMG: coarsest level too large");
   H->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (std::getenv("LLFEM_AMG_TRACE"))
+    std::fprintf(stderr, "[amg]   index checks %.1f ms\n",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count());
   return LLFEM_OK;
 }
 
