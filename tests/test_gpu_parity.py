@@ -139,6 +139,22 @@ def test_fp32_integration_error(gpu_ctx):
     assert 0 < rel < 1e-6
 
 
+def test_fp64_fp32_alternation_on_one_mesh(gpu_ctx):
+    """FP64 assemblies write only the FP64 part of the geometry records; an
+    FP32-integration assembly on the same mesh writes its FP32 copy again, so
+    FP64, FP32, FP64 in a row give the same bits as each alone."""
+    m = M.structured_box_mesh(lz=2, cells=(3, 3, 6), elem="tet", order=2)
+    tab = R.rule_tables(11, 2)
+    dev = _dev(gpu_ctx, m)
+    v1, _ = dev.assemble(_lib.OP_STIFFNESS, tab)
+    v32, _ = dev.assemble(_lib.OP_STIFFNESS, tab, precision=_lib.FP32_INT_FP64_SUM)
+    v2, _ = dev.assemble(_lib.OP_STIFFNESS, tab)
+    w32, _ = _dev(gpu_ctx, m).assemble(_lib.OP_STIFFNESS, tab, precision=_lib.FP32_INT_FP64_SUM)
+    assert np.array_equal(v1, v2)
+    assert np.array_equal(v32, w32)
+    assert 0 < np.linalg.norm(v32 - v1) / np.linalg.norm(v1) < 1e-6
+
+
 @pytest.mark.parametrize("elem,order,deg", [("hex", 1, 3), ("tet", 2, 5), ("hex", 2, 5),
                                             ("prism", 1, 3)])
 def test_mass_parity(gpu_ctx, elem, order, deg):
