@@ -171,8 +171,11 @@ LLFEM_API int llfem_assemble(llfem_mesh* mesh, const llfem_op_desc* desc, double
 
 /* ---- load vector -------------------------------------------------------
  * Replaces the traction branch of loadVector(problem, [load(name, fx,fy,fz)])
- * (src/linear.jl:3088-3434): f_a += sum_q N_a t |t1 x t2| w_q over the face
- * elements (face_conn [F][nf]); f_out is [pdim*N], zero-initialised here. */
+ * (src/linear.jl:3088-3434): f_a += sum_q N_a t Ja w_q over the boundary
+ * elements face_conn [F][nf] of a surface, curve or point group, with
+ * Ja = |t1 x t2| for surface elements, |t1| for curve (line) elements and 1
+ * for point elements (src/linear.jl:3378-3388); f_out is [pdim*N],
+ * zero-initialised here.  face_type is gmsh's element type code. */
 LLFEM_API int llfem_load_traction(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
                         int64_t F, int32_t one_based, int32_t q, const double* weights,
                         const double* N_ref, const double* dN_ref, const double traction[3],
@@ -189,8 +192,9 @@ LLFEM_API int llfem_face_points(llfem_mesh* mesh, int32_t face_type, int32_t nf,
                                 double* xyz_out);
 
 /* Traction given per face Gauss point, t_gp [F][q][3]:
- * f_a += sum_q N_a t_q |t1 x t2| w_q (src/linear.jl:3378-3413), summed per
- * node in ascending face order; f_out is [pdim*N], zero-initialised here. */
+ * f_a += sum_q N_a t_q Ja w_q (src/linear.jl:3378-3413), Ja as for
+ * llfem_load_traction, summed per node in ascending face order; f_out is
+ * [pdim*N], zero-initialised here. */
 LLFEM_API int llfem_load_traction_gp(llfem_mesh* mesh, int32_t face_type, int32_t nf, const int64_t* face_conn,
                                      int64_t F, int32_t one_based, int32_t q, const double* weights,
                                      const double* N_ref, const double* dN_ref, const double* t_gp,

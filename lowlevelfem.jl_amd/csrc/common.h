@@ -210,7 +210,9 @@ inline int etype_dim(int et) {
   switch (et) {
     case 4: case 5: case 6: case 11: case 12: case 92: case 93: return 3;
     case 2: case 3: case 9: case 10: case 36: case 37: return 2;
-    default: return 0;
+    case 1: case 8: case 26: case 27: return 1;
+    case 15: return 0;
+    default: return -1;
   }
 }
 

@@ -1,10 +1,12 @@
-// Load vectors: traction and pressure on a set of 2D face elements, body
-// force on the mesh's volume elements.
+// Load vectors: traction on a set of boundary elements (surface, curve or
+// point elements), pressure on a set of 2D face elements, body force on the
+// mesh's volume elements.
 //
 // Reference: loadVector(problem, [load(name; fx, fy, fz)]) (src/linear.jl:3088-3434):
-// per face element and Gauss point, Ja = |t1 x t2| from the face Jacobian
-// columns (src/linear.jl:3378-3384), f1 += H1' * t * Ja * w, then
-// fp[nn2] += f1 in element order (src/linear.jl:3408-3413).
+// per boundary element and Gauss point, Ja = |t1 x t2| on a surface,
+// |t1| on a curve and 1 at a point (src/linear.jl:3378-3388), from the
+// element's Jacobian columns, f1 += H1' * t * Ja * w, then fp[nn2] += f1 in
+// element order (src/linear.jl:3408-3413).
 //
 // Device version: one thread per face element computes the scalar nodal
 // integrals s_a = sum_q N_a Ja w (the traction is uniform over the face);
@@ -26,7 +28,21 @@
 
 namespace {
 
-__global__ void k_face_integrals(int64_t F, int nf, int q, int64_t N, const double* __restrict__ xyz,
+// Measure of a boundary element of dimension fd at one point, from its
+// Jacobian columns t1 = dx/dxi, t2 = dx/deta (src/linear.jl:3378-3388 for a
+// 3D solid): |t1 x t2| on a surface, |t1| on a curve, 1 at a point.
+__device__ inline double face_measure(int fd, const double t1[3], const double t2[3]) {
+  if (fd == 2) {
+    const double xy = t1[0] * t2[1] - t1[1] * t2[0];
+    const double yz = t1[1] * t2[2] - t1[2] * t2[1];
+    const double zx = t1[2] * t2[0] - t1[0] * t2[2];
+    return sqrt(xy * xy + yz * yz + zx * zx);
+  }
+  if (fd == 1) return sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  return 1.0;
+}
+
+__global__ void k_face_integrals(int64_t F, int nf, int fd, int q, int64_t N, const double* __restrict__ xyz,
                                  const int32_t* __restrict__ fconn, const double* __restrict__ w,
                                  const double* __restrict__ Nref, const double* __restrict__ dN,
                                  double* __restrict__ s) {
@@ -44,10 +60,7 @@ __global__ void k_face_integrals(int64_t F, int nf, int q, int64_t N, const doub
         t2[i] += x[i] * g1;
       }
     }
-    const double xy = t1[0] * t2[1] - t1[1] * t2[0];
-    const double yz = t1[1] * t2[2] - t1[2] * t2[1];
-    const double zx = t1[2] * t2[0] - t1[0] * t2[2];
-    const double Ja = sqrt(xy * xy + yz * yz + zx * zx);
+    const double Ja = face_measure(fd, t1, t2);
     for (int a = 0; a < nf; ++a) s[e * nf + a] += Nref[(size_t)p * nf + a] * Ja * w[p];
   }
 }
@@ -87,7 +100,7 @@ __global__ void k_face_points(int64_t F, int nf, int q, int64_t N, const double*
 
 // Per face element and node a: s3[a][c] = sum_q N_a t_c(q) Ja w_q, the
 // traction given per Gauss point (t [F][q][3]).
-__global__ void k_face_integrals_gp(int64_t F, int nf, int q, int64_t N, const double* __restrict__ xyz,
+__global__ void k_face_integrals_gp(int64_t F, int nf, int fd, int q, int64_t N, const double* __restrict__ xyz,
                                     const int32_t* __restrict__ fconn, const double* __restrict__ w,
                                     const double* __restrict__ Nref, const double* __restrict__ dN,
                                     const double* __restrict__ tgp, double* __restrict__ s3) {
@@ -105,10 +118,7 @@ __global__ void k_face_integrals_gp(int64_t F, int nf, int q, int64_t N, const d
         t2[i] += x[i] * g1;
       }
     }
-    const double xy = t1[0] * t2[1] - t1[1] * t2[0];
-    const double yz = t1[1] * t2[2] - t1[2] * t2[1];
-    const double zx = t1[2] * t2[0] - t1[0] * t2[2];
-    const double Ja = sqrt(xy * xy + yz * yz + zx * zx);
+    const double Ja = face_measure(fd, t1, t2);
     const double* tq = tgp + (e * q + p) * 3;
     for (int a = 0; a < nf; ++a) {
       const double c = Nref[(size_t)p * nf + a] * Ja * w[p];
@@ -296,9 +306,10 @@ struct FacePrep {
 
 int prepare_faces(const llfem_mesh* m, int32_t face_type, int32_t nf, const int64_t* face_conn, int64_t F,
                   int32_t one_based, int32_t q, FacePrep* P) {
-  if (etype_dim(face_type) != 2)
-    return fail(LLFEM_EUNSUP, "loadVector: face element type " + std::to_string(face_type) +
-                                  " is not a 2D element");
+  const int fd = etype_dim(face_type);
+  if (fd < 0 || fd > 2)
+    return fail(LLFEM_EUNSUP, "loadVector: boundary element type " + std::to_string(face_type) +
+                                  " is not a surface, curve or point element");
   if (nf <= 0 || q <= 0 || F < 0) return fail(LLFEM_EINVAL, "loadVector: bad face-set sizes");
   const int64_t base = one_based ? 1 : 0;
   P->fc.resize((size_t)F * nf);
@@ -329,34 +340,17 @@ extern "C" int llfem_load_traction(llfem_mesh* m, int32_t face_type, int32_t nf,
                                    const double traction[3], double* f_out) {
   if (!m || !face_conn || !weights || !N_ref || !dN_ref || !traction || !f_out)
     return fail(LLFEM_EINVAL, "llfem_load_traction: NULL argument");
-  if (etype_dim(face_type) != 2)
-    return fail(LLFEM_EUNSUP, "loadVector: face element type " + std::to_string(face_type) +
-                                  " is not a 2D element");
-  if (nf <= 0 || q <= 0 || F < 0) return fail(LLFEM_EINVAL, "llfem_load_traction: bad sizes");
+  FacePrep P;
+  LLFEM_TRY(prepare_faces(m, face_type, nf, face_conn, F, one_based, q, &P));
   llfem_ctx* ctx = m->ctx;
   LLFEM_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int64_t N = m->N;
-  const int64_t base = one_based ? 1 : 0;
-  // face-node incidences in ascending face order (host; face sets are 2D)
-  std::vector<int32_t> fc((size_t)F * nf);
-  for (int64_t i = 0; i < F * nf; ++i) {
-    int64_t v = face_conn[i] - base;
-    if (v < 0 || v >= N) return fail(LLFEM_EINVAL, "loadVector: face node out of range");
-    fc[i] = (int32_t)v;
-  }
-  std::vector<int32_t> slot_order((size_t)F * nf);
-  std::iota(slot_order.begin(), slot_order.end(), 0);
-  std::stable_sort(slot_order.begin(), slot_order.end(),
-                   [&](int32_t x, int32_t y) { return fc[x] < fc[y]; });
-  std::vector<int32_t> nodes, ptr{0};
-  for (size_t k = 0; k < slot_order.size(); ++k) {
-    if (k == 0 || fc[slot_order[k]] != fc[slot_order[k - 1]]) {
-      if (k) ptr.push_back((int32_t)k);
-      nodes.push_back(fc[slot_order[k]]);
-    }
-  }
-  ptr.push_back((int32_t)slot_order.size());
+  const int fd = etype_dim(face_type);
+  const std::vector<int32_t>& fc = P.fc;
+  const std::vector<int32_t>& nodes = P.nodes;
+  const std::vector<int32_t>& ptr = P.ptr;
+  const std::vector<int32_t>& slot_order = P.slots;
   const int64_t nn = (int64_t)nodes.size();
 
   int32_t *d_fc = nullptr, *d_nodes = nullptr, *d_ptr = nullptr, *d_slots = nullptr;
@@ -385,7 +379,7 @@ extern "C" int llfem_load_traction(llfem_mesh* m, int32_t face_type, int32_t nf,
     if (e != hipSuccess) { s = fail(LLFEM_EHIP, std::string("loadVector upload: ") + hipGetErrorString(e)); break; }
     if (F > 0) {
       k_face_integrals<<<(unsigned)((F + 127) / 128), 128, 0, st>>>(
-          F, nf, q, N, m->d_xyz, d_fc, d_tab, d_tab + q, d_tab + q + (size_t)q * nf, d_s);
+          F, nf, fd, q, N, m->d_xyz, d_fc, d_tab, d_tab + q, d_tab + q + (size_t)q * nf, d_s);
       k_face_gather<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(nn, d_nodes, d_ptr, d_slots, d_s,
                                                                   traction[0], traction[1],
                                                                   traction[2], d_f);
@@ -475,7 +469,7 @@ extern "C" int llfem_load_traction_gp(llfem_mesh* m, int32_t face_type, int32_t 
     if (e != hipSuccess) { s = fail(LLFEM_EHIP, std::string("loadVector upload: ") + hipGetErrorString(e)); break; }
     if (F > 0) {
       k_face_integrals_gp<<<(unsigned)((F + 127) / 128), 128, 0, st>>>(
-          F, nf, q, N, m->d_xyz, d_fc, d_tab, d_tab + q, d_tab + q + (size_t)q * nf, d_t, d_s3);
+          F, nf, etype_dim(face_type), q, N, m->d_xyz, d_fc, d_tab, d_tab + q, d_tab + q + (size_t)q * nf, d_t, d_s3);
       k_face_gather3<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(nn, d_nodes, d_ptr, d_slots, d_s3, d_f);
     }
     e = hipGetLastError();
@@ -623,6 +617,8 @@ extern "C" int llfem_load_pressure(llfem_mesh* m, int32_t face_type, int32_t nf,
                                    const double* p_nodal, double* f_out) {
   if (!m || !face_conn || !weights || !N_ref || !dN_ref || !dN_nodes || !f_out)
     return fail(LLFEM_EINVAL, "llfem_load_pressure: NULL argument");
+  if (etype_dim(face_type) != 2)
+    return fail(LLFEM_EINVAL, "loadVector: pressure can be given on a surface of a 3D solid.");
   FacePrep P;
   LLFEM_TRY(prepare_faces(m, face_type, nf, face_conn, F, one_based, q, &P));
   llfem_ctx* ctx = m->ctx;

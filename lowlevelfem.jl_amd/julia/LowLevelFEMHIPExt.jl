@@ -251,7 +251,9 @@ end
 
 # loadVector(problem, loads) for a 3D solid (src/linear.jl:3088-3434): on a
 # surface group, a traction (fx, fy, fz) or a pressure p (src/linear.jl:3196-3224;
-# p replaces fx, fy, fz); on a volume group, a body force.  Components and p
+# p replaces fx, fy, fz); on a curve group a line load and on a point group a
+# concentrated force (the library takes |dx/dxi| or 1 as the measure by the
+# element type, src/linear.jl:3385-3388); on a volume group, a body force.  Components and p
 # are numbers or functions of (x, y, z): traction and body-force functions
 # are evaluated at the Gauss points the library returns, p at the nodes
 # (scalarField, src/general.jl:4477-4515).
@@ -278,7 +280,7 @@ function gpu_loadVector(problem::Problem, loads)
                 end
                 continue
             end
-            edim == 2 || error("loadVector: only surface and volume groups are on the GPU path")
+            p === nothing || edim == 2 || error("loadVector: pressure can be given on a surface of a 3D solid.")
             ets2, _, ents = gmsh.model.mesh.getElements(edim, etag)
             for (et, enodes) in zip(ets2, ents)
                 _, _, order, nf, lnc, _ = gmsh.model.mesh.getElementProperties(et)

@@ -11,7 +11,7 @@ hot path (LowLevelFEM.jl 1.13.31):
     VectorField          src/general.jl:1554-1563 (container only)
     stiffnessMatrix      src/linear.jl:70-87 -> stiffnessMatrixSolid :190-330
     massMatrix           src/linear.jl:1843-2004
-    loadVector           src/linear.jl:3088-3434 (traction, pressure, body force)
+    loadVector           src/linear.jl:3088-3434 (traction, line and point loads, pressure, body force)
     solveDisplacement    src/linear.jl:3711-3743
     displacementConstraint / load / BoundaryCondition / LoadCondition
                          src/general.jl:2480-2605, 4179-4233
@@ -558,8 +558,10 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
     """loadVector(problem, loads) (src/linear.jl:3088-3434) for a 3D solid:
     on a surface group, traction load(name; fx, fy, fz) or pressure
     load(name; p) (src/linear.jl:3196-3224, p replaces fx, fy, fz); on a
-    volume group, a body force (src/linear.jl:3054-3068).  Components and p
-    are numbers or functions of (x, y, z): traction and body-force
+    curve group, a line load (force per length, Ja = |dx/dxi|), and on a
+    point group, a concentrated force (Ja = 1) (src/linear.jl:3385-3388);
+    on a volume group, a body force (src/linear.jl:3054-3068).  Components
+    and p are numbers or functions of (x, y, z): traction and body-force
     functions are evaluated at the Gauss points the library returns, p at
     the nodes (scalarField, src/general.jl:4477-4515)."""
     if problem.type == "dummy":
@@ -583,6 +585,8 @@ def loadVector(problem: Problem, loads) -> VectorField | None:
         name = ld.phName
         for fs in (problem.mesh.face_sets(name) if name in problem.mesh.faces else []):
             tab = _face_rule(fs)
+            if p is not None and tab.dim != 2:
+                raise ValueError("loadVector: pressure can be given on a surface of a 3D solid.")
             if p is not None:
                 dNn = refelem.basis(fs.etype, refelem.node_coords(fs.etype))[1]
                 if callable(p):

@@ -31,7 +31,7 @@ FACE_AXES = {"left": (0, 0), "right": (0, 1), "bottom": (1, 0),
 
 @dataclass
 class FaceSet:
-    etype: int                 # gmsh face element type
+    etype: int                 # gmsh boundary element type (surface, line or point)
     conn: np.ndarray           # (F, nf) 0-based node ids
 
 
@@ -48,7 +48,8 @@ class Mesh:
     coords: np.ndarray         # (N, 3) float64
     etype: int                 # gmsh volume element type (of the first block)
     conn: np.ndarray           # (E, npe) int64, 0-based node ids
-    faces: dict = field(default_factory=dict)   # name -> FaceSet, or a list of FaceSets of several types
+    faces: dict = field(default_factory=dict)   # boundary group (surface, curve or point) name -> FaceSet,
+                                                # or a list of FaceSets of several types
     name: str = "structured_box"
     volumes: dict = field(default_factory=dict)  # volume physical group -> element indices
     blocks: list = field(default_factory=list)   # further volume element types: Block per type
@@ -59,7 +60,8 @@ class Mesh:
         return [Block(self.etype, self.conn, self.volumes)] + list(self.blocks)
 
     def face_sets(self, name: str) -> list:
-        """The face sets of 2D physical group `name`, one per face element type."""
+        """The element sets of boundary physical group `name` (a surface,
+        curve or point group), one per element type."""
         if name not in self.faces:
             raise KeyError(f"Physical group '{name}' not found.")
         fs = self.faces[name]

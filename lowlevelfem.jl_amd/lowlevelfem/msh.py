@@ -48,7 +48,8 @@ import struct
 from functools import lru_cache
 
 # element types whose node order in refelem is gmsh's documented order
-GMSH_ORDERED = {2, 3, 4, 5, 6, 9, 10, 11, 12}
+# (points, lines of order 1-4, and the surface and volume types below)
+GMSH_ORDERED = {1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 15, 26, 27}
 # element types read through a node permutation (gmsh's recursive order)
 GMSH_PERMUTED = {36, 37, 92, 93}
 # nodes per element of gmsh's element type codes (binary element rows carry
@@ -363,17 +364,18 @@ def read_msh(path) -> Mesh:
                 volumes[name] = np.concatenate(idx)
         eblocks.append(Block(vt, conn, volumes))
 
+    # boundary groups: surfaces, curves and points (loads and supports)
     faces = {}
     for (dim, ptag), name in names.items():
-        if dim != 2:
+        if dim not in (0, 1, 2):
             continue
-        fb = [b for b in blocks if b[0] == 2 and ptag in ent_phys.get((2, b[1]), [])]
+        fb = [b for b in blocks if b[0] == dim and ptag in ent_phys.get((dim, b[1]), [])]
         if not fb:
             continue
         sets = []
         for ft in dict.fromkeys(b[2] for b in fb):
             if ft not in GMSH_ORDERED | GMSH_PERMUTED:
-                raise NotImplementedError(f"read_msh: face group '{name}' has element type {ft}")
+                raise NotImplementedError(f"read_msh: boundary group '{name}' has element type {ft}")
             sets.append(FaceSet(etype=ft, conn=np.ascontiguousarray(np.concatenate([b[3] for b in fb if b[2] == ft]))))
         faces[name] = sets[0] if len(sets) == 1 else sets
     b0 = eblocks[0]

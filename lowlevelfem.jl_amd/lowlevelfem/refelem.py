@@ -53,6 +53,9 @@ ELEMENT_TYPES = {
     12: ("hex27", 3, 2, 27, "hex"),
     36: ("quad16", 2, 3, 16, "quad"),
     37: ("quad25", 2, 4, 25, "quad"),
+    15: ("point", 0, 0, 1, "point"),
+    26: ("line4", 1, 3, 4, "line"),
+    27: ("line5", 1, 4, 5, "line"),
     92: ("hex64", 3, 3, 64, "hex"),
     93: ("hex125", 3, 4, 125, "hex"),
 }
@@ -184,6 +187,8 @@ def quadrature(family: str, degree: int):
     if family == "line":
         x, w = _gl(_npts(degree))
         return np.stack([x, 0 * x, 0 * x], axis=1), w
+    if family == "point":
+        return np.zeros((1, 3)), np.ones(1)
     if family == "tet":
         if degree <= 1:
             return np.array([[0.25, 0.25, 0.25]]), np.array([1.0 / 6.0])
@@ -318,6 +323,9 @@ def basis(etype: int, pts: np.ndarray):
         N[:] = L[:, perm]
         dN[:, :, 0] = dL[:, perm]
         return N, dN
+    if family == "point":
+        N[:] = 1.0
+        return N, dN
     raise ValueError(f"no basis for element type {etype}")
 
 
@@ -345,6 +353,8 @@ def node_coords(etype: int) -> np.ndarray:
     if family == "line":
         x = np.concatenate([[-1.0, 1.0], np.linspace(-1.0, 1.0, order + 1)[1:-1]])
         return np.stack([x, 0 * x, 0 * x], axis=1)
+    if family == "point":
+        return np.zeros((1, 3))
     raise ValueError(etype)
 
 

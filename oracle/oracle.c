@@ -409,11 +409,12 @@ API int oracle_mass_ijv(int64_t N, const double* xyz, int npe, int64_t E, const 
   return 0;
 }
 
-/* Traction part of loadVector (src/linear.jl:3246-3413) for a 2D face set in
- * 3D: per GP, Ja = |t1 x t2| from the face Jacobian columns
- * (src/linear.jl:3378-3384), f1 += H1' * t * Ja * w; fp[nn2] += f1.
- * dN is the face table [q][nf][3] (third component unused). */
-API int oracle_traction(int64_t N, const double* xyz, int nf, int64_t F, const int64_t* fconn, int q,
+/* Traction part of loadVector (src/linear.jl:3246-3413) for a boundary
+ * element set of dimension fd in 3D: per GP, Ja = |t1 x t2| for a surface
+ * (fd 2, src/linear.jl:3380-3384), |t1| for a curve (fd 1, :3385-3386) and 1
+ * for a point (fd 0, :3387-3388), from the element's Jacobian columns;
+ * f1 += H1' * t * Ja * w; fp[nn2] += f1.  dN is the element table [q][nf][3]. */
+API int oracle_traction(int64_t N, const double* xyz, int nf, int fd, int64_t F, const int64_t* fconn, int q,
                         const double* w, const double* Nref, const double* dN, const double* t,
                         double* f) {
   memset(f, 0, sizeof(double) * 3 * (size_t)N);
@@ -431,10 +432,15 @@ API int oracle_traction(int64_t N, const double* xyz, int nf, int64_t F, const i
           t2[i] += x[i] * dNq[3 * a + 1];
         }
       }
-      double xy = t1[0] * t2[1] - t1[1] * t2[0];
-      double yz = t1[1] * t2[2] - t1[2] * t2[1];
-      double zx = t1[2] * t2[0] - t1[0] * t2[2];
-      double Ja = sqrt(xy * xy + yz * yz + zx * zx);
+      double Ja = 1.0;
+      if (fd == 2) {
+        double xy = t1[0] * t2[1] - t1[1] * t2[0];
+        double yz = t1[1] * t2[2] - t1[2] * t2[1];
+        double zx = t1[2] * t2[0] - t1[0] * t2[2];
+        Ja = sqrt(xy * xy + yz * yz + zx * zx);
+      } else if (fd == 1) {
+        Ja = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+      }
       for (int a = 0; a < nf; ++a)
         for (int c = 0; c < 3; ++c) f1[3 * a + c] += Nref[k * nf + a] * t[c] * Ja * w[k];
     }

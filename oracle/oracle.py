@@ -40,7 +40,7 @@ def lib():
         L.oracle_stiffness_csc.restype = I
         L.oracle_mass_ijv.argtypes = [I64, P, I, I64, P, I, P, P, P, D, P, P, P]
         L.oracle_mass_ijv.restype = I
-        L.oracle_traction.argtypes = [I64, P, I, I64, P, I, P, P, P, P, P]
+        L.oracle_traction.argtypes = [I64, P, I, I, I64, P, I, P, P, P, P, P]
         L.oracle_traction.restype = I
         L.oracle_pcg.argtypes = [I64, P, P, P, P, P, P, D, I64, P, P]
         L.oracle_pcg.restype = I64
@@ -144,12 +144,13 @@ def mass_legacy(mesh, tab, rho=7.85e-9, lumped=False):
 
 def traction(mesh, faceset, tab, t):
     """loadVector(problem, [load(name; fx,fy,fz)]) traction branch
-    (src/linear.jl:3246-3413)."""
+    (src/linear.jl:3246-3413) on a surface, curve or point element set
+    (tab.dim = 2, 1, 0)."""
     xyz, fc = _c(mesh.coords, np.float64), _c(faceset.conn, np.int64)
     w, Nr, dN = _c(tab.weights, np.float64), _c(tab.N, np.float64), _c(tab.dN, np.float64)
     f = np.empty(3 * mesh.N)
     tt = _c(t, np.float64)
-    lib().oracle_traction(mesh.N, _p(xyz), fc.shape[1], fc.shape[0], _p(fc), tab.q, _p(w), _p(Nr),
+    lib().oracle_traction(mesh.N, _p(xyz), fc.shape[1], tab.dim, fc.shape[0], _p(fc), tab.q, _p(w), _p(Nr),
                           _p(dN), _p(tt), _p(f))
     return f
 
@@ -165,7 +166,8 @@ def traction_fn(mesh, faceset, tab, fns):
     """loadVector with function-valued components (fx, fy, fz callables of
     x, y, z, or numbers; None = 0), per face element and Gauss point as
     src/linear.jl:3277-3408 does it: t from _loadvec_helper (:3288-3299),
-    Ja = |t1 x t2| (:3378-3384), f1 += H1' t Ja w, fp[nn2] += f1 in element
+    Ja = |t1 x t2| on a surface (:3380-3384), |t1| on a curve (:3385-3386),
+    1 at a point (:3387-3388), f1 += H1' t Ja w, fp[nn2] += f1 in element
     order.  Plain numpy loops over the face elements (small meshes)."""
     P = face_points(mesh, faceset, tab)
     F, q = P.shape[:2]
@@ -184,7 +186,8 @@ def traction_fn(mesh, faceset, tab, fns):
         for j in range(q):
             t1 = tab.dN[j, :, 0] @ X
             t2 = tab.dN[j, :, 1] @ X
-            Ja = np.linalg.norm(np.cross(t1, t2))
+            Ja = (np.linalg.norm(np.cross(t1, t2)) if tab.dim == 2
+                  else np.linalg.norm(t1) if tab.dim == 1 else 1.0)
             f1 += np.outer(tab.N[j], T[e, j]) * Ja * tab.weights[j]
         for a, v in enumerate(nodes):
             f[3 * v:3 * v + 3] += f1[a]

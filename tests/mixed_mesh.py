@@ -49,30 +49,43 @@ def _gmsh_conn(etype, conn):
 
 def write_msh(mesh, path, binary=False):
     """MSH 4.1, ASCII or binary (little-endian, data size 8): one volume
-    entity per (volume group, element type), one surface entity per (face
-    group, face type); node tags 1..N; element nodes in gmsh's order."""
+    entity per (volume group, element type), one point, curve or surface
+    entity per (boundary group, element type) by the element type's
+    dimension; node tags 1..N; element nodes in gmsh's order."""
+    from lowlevelfem import refelem as R
     groups = list(dict.fromkeys(g for b in mesh.element_blocks() for g in b.volumes))
     fnames = list(mesh.faces)
     ptag = {("v", g): i + 1 for i, g in enumerate(groups)}
     ptag.update({("s", f): len(groups) + i + 1 for i, f in enumerate(fnames)})
     lo, hi = mesh.coords.min(axis=0), mesh.coords.max(axis=0)
-    vol_ents, surf_ents = [], []
+    # per dimension 0..3: entities (tag, physical tag, element type, connectivity)
+    ents_by_dim = [[], [], [], []]
+    fdim = {}
+    for f in fnames:
+        for fs in mesh.face_sets(f):
+            d = R.element_properties(fs.etype)[1]
+            fdim[f] = d
+            ents_by_dim[d].append((len(ents_by_dim[d]) + 1, ptag[("s", f)], fs.etype, _gmsh_conn(fs.etype, fs.conn)))
     for g in groups:
         for b in mesh.element_blocks():
             if g in b.volumes:
-                vol_ents.append((len(vol_ents) + 1, ptag[("v", g)], b.etype, _gmsh_conn(b.etype, b.conn[b.volumes[g]])))
-    for f in fnames:
-        for fs in mesh.face_sets(f):
-            surf_ents.append((len(surf_ents) + 1, ptag[("s", f)], fs.etype, _gmsh_conn(fs.etype, fs.conn)))
-    ents = [(2, t, et, c) for t, _, et, c in surf_ents] + [(3, t, et, c) for t, _, et, c in vol_ents]
+                ents_by_dim[3].append((len(ents_by_dim[3]) + 1, ptag[("v", g)], b.etype,
+                                       _gmsh_conn(b.etype, b.conn[b.volumes[g]])))
+    ents = [(d, t, et, c) for d in range(4) for t, _, et, c in ents_by_dim[d]]
     ne = sum(c.shape[0] for *_, c in ents)
     N = mesh.N
-    names = [f'3 {ptag[("v", g)]} "{g}"' for g in groups] + [f'2 {ptag[("s", f)]} "{f}"' for f in fnames]
+    names = [f'3 {ptag[("v", g)]} "{g}"' for g in groups] + [f'{fdim[f]} {ptag[("s", f)]} "{f}"' for f in fnames]
     if not binary:
         bbox = " ".join(f"{v:.17g}" for v in list(lo) + list(hi))
         out = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(ptag))] + names
-        out += ["$EndPhysicalNames", "$Entities", f"0 0 {len(surf_ents)} {len(vol_ents)}"]
-        out += [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in surf_ents] + [f"{t} {bbox} 1 {p} 0" for t, p, _, _ in vol_ents]
+        out += ["$EndPhysicalNames", "$Entities", " ".join(str(len(e)) for e in ents_by_dim)]
+        for d in range(4):
+            for t, p, _, c in ents_by_dim[d]:
+                if d == 0:
+                    x = mesh.coords[c[0, 0]]
+                    out.append(f"{t} " + " ".join(f"{v:.17g}" for v in x) + f" 1 {p}")
+                else:
+                    out.append(f"{t} {bbox} 1 {p} 0")
         out += ["$EndEntities", "$Nodes", f"1 {N} 1 {N}", f"3 1 0 {N}"]
         out += [str(i + 1) for i in range(N)] + [" ".join(f"{v:.17g}" for v in x) for x in mesh.coords]
         out += ["$EndNodes", "$Elements", f"{len(ents)} {ne} 1 {ne}"]
@@ -92,9 +105,13 @@ def write_msh(mesh, path, binary=False):
     d8 = lambda *v: struct.pack(f"<{len(v)}d", *v)
     parts = [b"$MeshFormat\n4.1 1 8\n", i4(1), b"\n$EndMeshFormat\n$PhysicalNames\n",
              ("\n".join([str(len(ptag))] + names) + "\n").encode(), b"$EndPhysicalNames\n$Entities\n",
-             q(0, 0, len(surf_ents), len(vol_ents))]
-    for t, p, _, _ in surf_ents + vol_ents:
-        parts += [i4(t), d8(*lo, *hi), q(1), i4(p), q(0)]
+             q(*(len(e) for e in ents_by_dim))]
+    for d in range(4):
+        for t, p, _, c in ents_by_dim[d]:
+            if d == 0:
+                parts += [i4(t), d8(*mesh.coords[c[0, 0]]), q(1), i4(p)]
+            else:
+                parts += [i4(t), d8(*lo, *hi), q(1), i4(p), q(0)]
     parts += [b"\n$EndEntities\n$Nodes\n", q(1, N, 1, N), i4(3, 1, 0), q(N),
               np.arange(1, N + 1, dtype="<u8").tobytes(), np.ascontiguousarray(mesh.coords, dtype="<f8").tobytes(),
               b"\n$EndNodes\n$Elements\n", q(len(ents), ne, 1, ne)]
@@ -108,3 +125,32 @@ def write_msh(mesh, path, binary=False):
     parts.append(b"\n$EndElements\n")
     with open(path, "wb") as fh:
         fh.write(b"".join(parts))
+
+
+LINE_TYPE_BY_ORDER = {1: 1, 2: 8, 3: 26, 4: 27}
+
+
+def with_edge_and_corner(mesh):
+    """The box `mesh` with three more boundary groups: "edge" (the line
+    elements of the order of the mesh along the box edge y = y_min,
+    z = z_min, vertices first, then the interior nodes from the first vertex
+    to the second), "corner" (a point element at the node with the largest
+    x + y + z) and "corners2" (point elements at two nodes, the second one
+    listed twice)."""
+    from lowlevelfem import refelem as R
+    order = R.element_properties(mesh.etype)[2]
+    x = mesh.coords
+    lo = x.min(axis=0)
+    on = np.flatnonzero((np.abs(x[:, 1] - lo[1]) < 1e-12) & (np.abs(x[:, 2] - lo[2]) < 1e-12))
+    on = on[np.argsort(x[on, 0])]
+    ne = (on.size - 1) // order
+    assert ne * order + 1 == on.size, "edge nodes do not split into elements"
+    rows = [[on[order * e], on[order * (e + 1)]] + [on[order * e + k] for k in range(1, order)] for e in range(ne)]
+    corner = int(np.argmax(x.sum(axis=1)))
+    other = int(np.argmin(x.sum(axis=1)))
+    faces = dict(mesh.faces)
+    faces["edge"] = M.FaceSet(LINE_TYPE_BY_ORDER[order], np.array(rows, dtype=np.int64))
+    faces["corner"] = M.FaceSet(15, np.array([[corner]], dtype=np.int64))
+    faces["corners2"] = M.FaceSet(15, np.array([[corner], [other], [other]], dtype=np.int64))
+    return M.Mesh(mesh.coords, mesh.etype, mesh.conn, faces=faces, name=mesh.name,
+                  volumes=mesh.volumes, blocks=mesh.blocks)

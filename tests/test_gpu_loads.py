@@ -3,7 +3,8 @@
 test_oracle_loads.py), and the mirror's loadVector / ∫(Pu ⋅ g) on top of
 them, including the reference's resultant check of test/test_dsl.jl:50-66
 (∫(Pu ⋅ [1, 1, 1]) over the unit cube and over its face "right" both sum to
-[1, 1, 1]).  Loads are FP64 sums of a few terms per node: the bar is 1e-13
+[1, 1, 1]); line loads on curve groups and point loads on point groups
+(oracle.traction with Ja = |dx/dxi| or 1, pinned in test_oracle_loads.py).  Loads are FP64 sums of a few terms per node: the bar is 1e-13
 relative to the largest entry; repeated calls are bit-identical."""
 import sys
 
@@ -132,3 +133,73 @@ def test_loadvector_body_pressure_and_function(gpu_ctx):
     assert close(f.a[:, 0], want)
     with pytest.raises(ValueError, match="pressure can be given on a surface"):
         L.loadVector(P, [L.load("body", p=1.0)])
+
+
+# ---- line and point loads (src/linear.jl:3385-3388) ------------------------
+EDGE = [("hex8", dict(cells=(5, 2, 2), lx=2.0)), ("tet10", dict(n=2, elem="tet", order=2)),
+        ("hex64", dict(n=2, order=3)), ("hex125", dict(n=2, order=4))]
+
+
+@pytest.mark.parametrize("label,kw", EDGE, ids=[e[0] for e in EDGE])
+def test_line_and_point_loads_match_oracle(gpu_ctx, label, kw):
+    """Line loads on the box edge (line2 ... line5 elements, their interior
+    nodes moved along the edge so |dx/dxi| varies) and point loads: constant
+    and function-valued, against the oracle; the resultant of a uniform
+    line load is q L."""
+    from mixed_mesh import with_edge_and_corner
+    say(f"[edge] {label}")
+    m = with_edge_and_corner(M.structured_box_mesh(**kw))
+    fs = m.faces["edge"]
+    order = R.element_properties(fs.etype)[2]
+    if order > 1:            # interior edge nodes slide along x: non-affine x(xi)
+        x = m.coords.copy()
+        inner = fs.conn[:, 2:].ravel()
+        x[inner, 0] += 0.08 * (x[fs.conn[:, 1], 0] - x[fs.conn[:, 0], 0]).repeat(order - 1) / order
+        m = M.Mesh(x, m.etype, m.conn, m.faces)
+    tab = R.rule_tables(fs.etype, 2 * order + 1)
+    dev = _lib.DeviceMesh(gpu_ctx, m.coords, m.etype, m.conn)
+    t = np.array([1.5, -0.5, 2.0])
+    f = dev.traction(fs.etype, fs.conn, tab, t)
+    assert close(f, O.traction(m, fs, tab, t))
+    Lx = m.coords[:, 0].max() - m.coords[:, 0].min()
+    assert np.allclose(f.reshape(-1, 3).sum(axis=0), t * Lx, rtol=1e-13, atol=0)
+    P = dev.face_points(fs.etype, fs.conn, tab)
+    assert close(P, O.face_points(m, fs, tab), 1e-15)
+    fns = [lambda x, y, z: x * x, None, lambda x, y, z: 1.0 - x]
+    ref, T = O.traction_fn(m, fs, tab, fns)
+    assert close(dev.traction_gp(fs.etype, fs.conn, tab, T), ref)
+    for name in ("corner", "corners2"):
+        ps = m.faces[name]
+        ptab = R.rule_tables(15, 1)
+        fp = dev.traction(15, ps.conn, ptab, t)
+        assert np.array_equal(fp, O.traction(m, ps, ptab, t))
+    dev.close()
+
+
+@pytest.mark.parametrize("binary", [False, True], ids=["ascii", "binary"])
+def test_loadvector_on_curve_and_point_groups_from_msh(gpu_ctx, tmp_path, binary):
+    """The mirror's loadVector on a curve group (line load, one component a
+    function of position) and on point groups, read back from an MSH file
+    with physical curves and points; a point group also serves as a
+    support; pressure on a curve is refused as in the reference."""
+    from mixed_mesh import with_edge_and_corner, write_msh
+    say(f"[edge-msh] binary={binary}")
+    m0 = with_edge_and_corner(M.structured_box_mesh(n=2, elem="tet", order=2))
+    m0.volumes = {"body": np.arange(m0.E)}
+    path = str(tmp_path / ("edges.msh" + ("b" if binary else "")))
+    write_msh(m0, path, binary=binary)
+    m = L.read_msh(path)
+    assert m.faces["edge"].etype == 8 and m.faces["corner"].etype == 15
+    assert np.array_equal(m.faces["corners2"].conn, m0.faces["corners2"].conn)
+    P = L.Problem([L.Material("body", E=2e5, nu=0.3)])
+    f = L.loadVector(P, [L.load("edge", fx=2.0, fy=lambda x, y, z: 1.0 + x),
+                         L.load("corners2", fz=-3.0)])
+    fs, ps = m.faces["edge"], m.faces["corners2"]
+    tab = R.rule_tables(8, 5)
+    want, _ = O.traction_fn(m, fs, tab, [2.0, lambda x, y, z: 1.0 + x, None])
+    want = want + O.traction(m, ps, R.rule_tables(15, 1), [0.0, 0.0, -3.0])
+    assert close(f.a[:, 0], want)
+    assert np.isclose(f.a[:, 0].reshape(-1, 3)[:, 0].sum(), 2.0, rtol=1e-13)
+    assert np.array_equal(L.nodesOnPhysicalGroup(P, "corner"), m0.faces["corner"].conn[0])
+    with pytest.raises(ValueError, match="pressure can be given on a surface"):
+        L.loadVector(P, [L.load("edge", p=1.0)])

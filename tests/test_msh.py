@@ -184,3 +184,22 @@ def test_high_order_hex_round_trip(tmp_path, order, binary):
     cp, rv = O.csc_pattern(m)
     D = hooke_symgrad_D()
     assert np.array_equal(O.stiffness_csc(r, tab, D, *O.csc_pattern(r)), O.stiffness_csc(m, tab, D, cp, rv))
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+@pytest.mark.parametrize("binary", [False, True], ids=["ascii", "binary"])
+def test_curve_and_point_groups_round_trip(tmp_path, order, binary):
+    """Physical curves (line2 / line3 / line5) and points (one point element,
+    and two with a node listed twice) are kept as boundary groups with their
+    element type and node order; their nodes are the group's nodes."""
+    from mixed_mesh import with_edge_and_corner, write_msh
+    m = with_edge_and_corner(M.structured_box_mesh(n=2, order=order))
+    m.volumes = {"body": np.arange(m.E)}
+    path = tmp_path / f"edges{order}.msh"
+    write_msh(m, path, binary=binary)
+    r = read_msh(path)
+    for name in ("edge", "corner", "corners2", "left"):
+        assert r.faces[name].etype == m.faces[name].etype
+        assert np.array_equal(r.faces[name].conn, m.faces[name].conn), name
+    assert np.array_equal(r.nodes_on("corner"), m.faces["corner"].conn[0])
+    assert r.nodes_on("edge").size == 2 * order + 1

@@ -14,7 +14,11 @@ oracle.pressure) by what the mathematics fixes, on the CPU:
 - pressure on a quarter-cylinder surface has resultant -p R L (1, 1, 0) up
   to the discretisation error, sign fixed by the faces' orientation;
 - on a closed box surface the corner and edge nodes average the normals of
-  their faces, and the resultant vanishes."""
+  their faces, and the resultant vanishes;
+- line loads (oracle.traction on line elements, Ja = |dx/dxi|): the
+  Newton-Cotes nodal weights on straight equispaced elements of order 1-4,
+  resultant q L and first moment q L^2 / 2 when the interior nodes are
+  moved; point loads (Ja = 1) are the force at the node."""
 import numpy as np
 
 from lowlevelfem import mesh as M
@@ -141,3 +145,71 @@ def test_closed_box_nodal_normals_average():
     corner = np.sign(m.coords - 0.5)
     assert np.allclose(tn, -corner / 3, rtol=0, atol=1e-15)
     assert np.allclose(f.reshape(-1, 3).sum(axis=0), 0.0, atol=1e-14)
+
+
+# ---- line and point loads (src/linear.jl:3385-3388) ------------------------
+def _segment_mesh(order, shift, L=3.0, d=(1.0, 2.0, 2.0)):
+    """One straight segment of length L along d in 3D as one line element of
+    the given order; the interior nodes sit at the equispaced positions plus
+    `shift` (fractions of L), so the parametrisation is not affine when
+    shift != 0.  Returns (mesh, faceset, arc-length coordinate per node)."""
+    u = np.asarray(d) / np.linalg.norm(d)
+    s = np.concatenate([[0.0, L], np.linspace(0.0, L, order + 1)[1:-1] + shift * L])
+    coords = np.array([0.5, -0.2, 1.0]) + s[:, None] * u[None, :]
+    et = {1: 1, 2: 8, 3: 26, 4: 27}[order]
+    fs = M.FaceSet(et, np.arange(order + 1, dtype=np.int64)[None, :])
+    return M.Mesh(coords, 4, np.zeros((0, 4), np.int64)), fs, s
+
+
+def test_line_load_textbook_nodal_values():
+    """Uniform line load q on a straight element with equispaced nodes: the
+    consistent nodal loads are q L times the Newton-Cotes weights (trapezoid,
+    Simpson, Simpson 3/8, Boole)."""
+    want = {1: [1 / 2, 1 / 2], 2: [1 / 6, 1 / 6, 2 / 3], 3: [1 / 8, 1 / 8, 3 / 8, 3 / 8],
+            4: [7 / 90, 7 / 90, 32 / 90, 12 / 90, 32 / 90]}
+    t = np.array([0.7, -1.1, 2.5])
+    for order, wts in want.items():
+        m, fs, s = _segment_mesh(order, 0.0)
+        f = O.traction(m, fs, R.rule_tables(fs.etype, 2 * order + 1), t).reshape(-1, 3)
+        assert np.allclose(f, 3.0 * np.outer(wts, t), rtol=1e-13, atol=1e-15), order
+
+
+def test_line_load_resultant_and_moment_on_curved_parametrisation():
+    """With the interior nodes moved along the segment, x(xi) is not affine
+    and |dx/dxi| varies; still sum_a f_a = q L and sum_a f_a s_a = q L^2 / 2
+    (the integrands are polynomials the rule integrates exactly), which a
+    missing or wrong Ja would break."""
+    for order, shift in ((2, 0.12), (3, 0.05), (1, 0.0)):
+        m, fs, s = _segment_mesh(order, shift)
+        f = O.traction(m, fs, R.rule_tables(fs.etype, 2 * order + 1), [2.0, 0.0, 0.0]).reshape(-1, 3)[:, 0]
+        assert abs(f.sum() - 2.0 * 3.0) < 1e-13
+        if order <= 2:
+            assert abs(f @ s - 2.0 * 3.0 ** 2 / 2) < 1e-12
+
+
+def test_line_load_function_of_position():
+    """q = s (arc length) on one line3 with its mid-node moved: sum f = L^2 / 2
+    and the function-valued path equals the constant one for a constant."""
+    m, fs, s = _segment_mesh(2, 0.1)
+    u = np.array([1.0, 2.0, 2.0]) / 3.0
+    x0 = np.array([0.5, -0.2, 1.0])
+    tab = R.rule_tables(fs.etype, 5)
+    f, T = O.traction_fn(m, fs, tab, [lambda x, y, z: float(np.dot([x, y, z] - x0, u)), None, None])
+    assert abs(f.reshape(-1, 3)[:, 0].sum() - 3.0 ** 2 / 2) < 1e-12
+    fc, _ = O.traction_fn(m, fs, tab, [0.5, None, -1.0])
+    assert np.allclose(fc, O.traction(m, fs, tab, [0.5, 0.0, -1.0]), rtol=0, atol=1e-15)
+
+
+def test_point_load_is_the_force_at_its_node():
+    """A point element carries the force itself (N = 1, Ja = 1, w = 1); a
+    node listed twice gets it twice."""
+    m = M.structured_box_mesh(n=2)
+    fs = M.FaceSet(15, np.array([[3], [7], [7]], dtype=np.int64))
+    tab = R.rule_tables(15, 1)
+    assert tab.q == 1 and tab.weights[0] == 1.0 and tab.N[0, 0] == 1.0
+    t = np.array([1.5, -2.0, 0.25])
+    f = O.traction(m, fs, tab, t).reshape(-1, 3)
+    want = np.zeros_like(f)
+    want[3] = t
+    want[7] = 2 * t
+    assert np.array_equal(f, want)

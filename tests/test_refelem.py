@@ -27,24 +27,26 @@ def _exact_monomial(family, a, b, c):
     if family == "prism":
         f = lambda k: 0.0 if k % 2 else 2.0 / (k + 1)
         return factorial(a) * factorial(b) / factorial(a + b + 2) * f(c)
+    if family == "line":
+        return (0.0 if a % 2 else 2.0 / (a + 1)) * (1.0 if b == c == 0 else 0.0)
     raise ValueError(family)
 
 
-@pytest.mark.parametrize("family,deg", [(f, d) for f in ("hex", "quad", "tet", "tri", "prism")
+@pytest.mark.parametrize("family,deg", [(f, d) for f in ("hex", "quad", "tet", "tri", "prism", "line")
                                         for d in (1, 2, 3, 5, 9)])
 def test_quadrature_exact(family, deg):
     pts, w = R.quadrature(family, deg)
     assert np.all(w > 0)
     assert abs(w.sum() - VOLUME[family]) < 1e-14
-    dims = 3 if family in ("hex", "tet", "prism") else 2
+    dims = 3 if family in ("hex", "tet", "prism") else 2 if family in ("quad", "tri") else 1
     for a, b, c in itertools.product(range(deg + 1), repeat=3):
-        if dims == 2 and c:
+        if (dims <= 2 and c) or (dims == 1 and b):
             continue
         if family in ("tet", "tri") and a + b + c > deg:
             continue
         if family == "prism" and (a + b > deg or c > deg):
             continue
-        if family in ("hex", "quad") and max(a, b, c) > deg:
+        if family in ("hex", "quad", "line") and max(a, b, c) > deg:
             continue
         got = np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
         assert abs(got - _exact_monomial(family, a, b, c)) < 1e-13, (a, b, c)
@@ -63,7 +65,8 @@ def test_basis_partition_and_interpolation(etype):
     # gradients reproduce linear functions: sum_a x_a dN_a = I
     G = np.einsum("qai,aj->qji", dN, X)
     eye = np.eye(3)[:dim, :dim]
-    assert np.abs(G[:, :dim, :dim] - eye[None]).max() < 1e-12
+    if dim:                                          # a point has no gradients
+        assert np.abs(G[:, :dim, :dim] - eye[None]).max() < 1e-12
 
 
 def test_gmsh_low_order_orderings():
