@@ -491,11 +491,13 @@ struct HostLevel {
   HVec<int32_t> gblk, gcol;
 };
 
-// Run f(lo, hi) over [0, n) split across host threads.
+// Run f(lo, hi) over [0, n) split across host threads; serial below `grain`
+// items (per-aggregate loops pass a small grain: on coarse levels there are
+// few aggregates, each with many members).
 template <class F>
-void parallel_for(int64_t n, F&& f) {
+void parallel_for(int64_t n, F&& f, int64_t grain = 4096) {
   const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), 16));
-  if (n < 4096 || nt == 1) {
+  if (n < grain || nt == 1) {
     f((int64_t)0, n);
     return;
   }
@@ -744,7 +746,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
       for (int k = 0; k < 36; ++k) Bc[36 * a + k] = R[k];
       for (int u = 0; u < 6; ++u) C.mask[6 * a + u] = dropped[u] ? 1 : 0;
     }
-  });
+  }, 64);
   tick("prolongator", tp);
   // per aggregate: neighbour aggregates (every fine block couples), then the
   // Galerkin contribution lists of its pairs, fine blocks in ascending order.
@@ -782,7 +784,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
       }
       nblk[a + 1] = c;
     }
-  });
+  }, 64);
   for (int64_t a = 0; a < na; ++a) {
     nbp[a + 1] += nbp[a];
     nblk[a + 1] += nblk[a];
@@ -823,7 +825,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
         }
       }
     }
-  });
+  }, 64);
   tick("galerkin lists", tp);
   // coarse pseudo-node pattern: 2A and 2A+1 both list [2B, 2B+1 for B in nb(A)]
   const int64_t npairs = nbp[na];
@@ -847,7 +849,7 @@ void coarsen(HostLevel& L, const int64_t* adj_ptr, const int32_t* adj, int ps, i
         }
       }
     }
-  });
+  }, 64);
   C.adj_ptr[2 * na] = 4 * nbp[na];
   tick("coarse pattern", tp);
 }
