@@ -373,8 +373,9 @@ def bench_batch(args):
     for kind, (elem, order) in W.KINDS.items():     # one tuning sample per element type
         sm = M.structured_box_mesh(n=24 if order == 1 else 12, elem=elem, order=order)
         r = _lib.DeviceMesh(ctx, sm.coords, sm.etype, sm.conn).autotune(_lib.OP_STIFFNESS, rules[kind][0])
-        tuned[kind] = {"mult": r.mult, "waves_per_block": r.waves_per_block,
-                       "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates}
+        tuned[kind] = {"mult": r.mult, "waves_per_block": r.waves_per_block, "streams": r.streams,
+                       "round_kernels": r.round_kernels, "best_ms": r.best_ms, "default_ms": r.default_ms,
+                       "candidates": r.candidates}
         log(f"autotune {kind}: {tuned[kind]}")
     t0 = time.perf_counter()
     devs = [_lib.DeviceMesh(ctx, m.coords, m.etype, m.conn) for m in meshes]

@@ -299,17 +299,21 @@ LLFEM_API int llfem_mesh_values(llfem_mesh* mesh, int32_t op, int32_t lumped, do
 /* ---- autotuning ------------------------------------------------------------
  * Times the panel assembly of `desc` on `sample` over a grid of launch
  * shapes (launch-width multiplier of the record planner x waves per block),
+ * then, for the fastest shape, over the number of streams the node-class
+ * launches of one assembly are spread over (1-3) x round kernels on/off;
  * keeps the fastest for the sample's element type in the context (meshes
  * created afterwards, and llfem_assemble / llfem_assemble_batch, use it) and
- * re-plans the sample.  No reference counterpart: the reference's
- * per-element loops have no launch shape. */
+ * re-plans the sample.  Values do not depend on the choice.  No reference
+ * counterpart: the reference's per-element loops have no launch shape. */
 typedef struct {
   int32_t elem_type;       /* gmsh element type code tuned */
   int32_t mult;            /* chosen launch-width multiplier */
   int32_t waves_per_block; /* chosen waves per block */
-  int32_t candidates;      /* shapes timed */
-  double best_ms;          /* device time of the chosen shape */
-  double default_ms;       /* device time of the default shape */
+  int32_t candidates;      /* configurations timed */
+  double best_ms;          /* device time of the chosen configuration */
+  double default_ms;       /* device time of the default configuration */
+  int32_t streams;         /* chosen stream count for the class launches */
+  int32_t round_kernels;   /* 1: step-unrolled round kernels where they exist */
 } llfem_tune_result;
 
 LLFEM_API int llfem_autotune(llfem_mesh* sample, const llfem_op_desc* desc, llfem_tune_result* out);

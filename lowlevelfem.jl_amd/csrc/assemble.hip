@@ -32,6 +32,9 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "common.h"
 
@@ -124,6 +127,50 @@ PanelFn select_rounds_real(int npe, int cnt, int ks_needed) {
 // busiest unit of these kernels (FP64 assembly of the tet10 beam 2.09 -> 2.04
 // ms; the FP32 kernels measured 2 % slower with it and keep the global table).
 // LLFEM_PANEL_TABLE_LDS=0 switches it off for A/B runs.
+// Waves per block (1..max_wpb) that keep the most waves of `fn` resident on
+// a CU: by LDS (c waves take c * wave_bytes + block_extra bytes) and by what
+// the runtime's occupancy calculator allows for the kernel's registers.  The
+// round kernels use up to ~240 VGPRs (two waves per SIMD): sized by LDS alone,
+// a 6-wave block left a CU with one block, 6 of its 8 wave slots.
+// Cached per (kernel, sizes).  LLFEM_PANEL_OCC=0 sizes by LDS alone (A/B).
+int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, size_t, size_t, int>, int> cache;
+  const char* env = std::getenv("LLFEM_PANEL_OCC");
+  const bool occ = !env || std::atoi(env) != 0;
+  const auto key = std::make_tuple((const void*)fn, wave_bytes, block_extra, max_wpb * (occ ? 1 : -1));
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  if (occ) (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCU);
+  int wpb = 0;
+  size_t best = 0;
+  for (int c = 1; c <= max_wpb && c * wave_bytes + block_extra <= kLdsPerCU; ++c) {
+    const size_t lds = c * wave_bytes + block_extra;
+    size_t blocks = kLdsPerCU / lds;
+    if (occ) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, 64 * c, lds) == hipSuccess && nb > 0)
+        blocks = std::min(blocks, (size_t)nb);
+    }
+    const size_t resident = (size_t)c * blocks;
+    // ties: with a per-block table in LDS, the block nearest 4 waves (1-wave
+    // blocks repeat the table fill per wave, 8-wave blocks leave a CU idle
+    // while their last wave finishes); without one, the smallest block
+    // (measured on the tet10 beam's FP64 and FP32 round kernels)
+    const bool table = block_extra > 64;
+    if (resident > best || (occ && table && resident == best && std::abs(c - 4) < std::abs(wpb - 4))) {
+      best = resident;
+      wpb = c;
+    }
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = wpb;
+  return wpb;
+}
+
 bool rounds_table_lds() {
   const char* e = std::getenv("LLFEM_PANEL_TABLE_LDS");
   return !e || std::atoi(e) != 0;
@@ -433,13 +480,17 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   else
     LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
   // Node classes write disjoint panels, so their launches may overlap: each
-  // launch goes to the least-loaded of two streams (largest work first; work
-  // = steps of the launch), which fills the tail of one class's launch with
-  // the next.  LLFEM_PANEL_STREAMS=n (1..3) sets the count.
+  // launch goes to the least-loaded of two streams (largest work first;
+  // work = steps of the launch), which fills the tail of one class's launch
+  // with the others.  A tuned count, or LLFEM_PANEL_STREAMS=n (1..3), sets it.
   static const std::vector<PanelLaunch> kNoLaunches;
   const std::vector<PanelLaunch>& plan = highorder ? kNoLaunches : m->launches;
+  const auto tuned = ctx->tune.find(m->etype);
   int nstreams = 2;
+  if (tuned != ctx->tune.end() && tuned->second.streams > 0) nstreams = std::min(3, tuned->second.streams);
   if (const char* env = std::getenv("LLFEM_PANEL_STREAMS")) nstreams = std::max(1, std::min(3, std::atoi(env)));
+  const bool rounds_on =
+      !std::getenv("LLFEM_PANEL_NO_ROUNDS") && !(tuned != ctx->tune.end() && tuned->second.rounds == 0);
   nstreams = std::min<int>(nstreams, (int)plan.size());
   std::vector<int> stream_of(plan.size(), 0);
   if (nstreams > 1) {
@@ -477,28 +528,25 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     size_t block_extra = 0;
     if (!std::getenv("LLFEM_PANEL_RUNTIME_STORES")) {   // A/B knob: the run-time store loop
       if (PanelFn f = select_fixed_stores(npe, m->map_bytes, op, affine, iso, fp32, L.max_r)) lfn = f;
-      if (!std::getenv("LLFEM_PANEL_NO_ROUNDS"))        // A/B knob: the step-unrolled kernel
+      if (rounds_on)                                    // A/B knob / tuned: the step-unrolled kernel
         if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) {
           lfn = f;
           if (rounds_table_lds() && !fp32) block_extra = 8 + 10 * sizeof(double) * npe * npe;
         }
     }
-    // waves per block: the size that packs the most waves into one CU's LDS
+    // waves per block: the size that keeps the most waves resident on a CU
     // (several small blocks beat one large block with LDS left over)
-    int wpb = 0;
-    size_t best = 0;
-    for (size_t c = 1; c <= max_wpb && c * wave_bytes + block_extra <= kLdsPerCU; ++c) {
-      const size_t resident = c * (kLdsPerCU / (c * wave_bytes + block_extra));
-      if (resident > best) { best = resident; wpb = (int)c; }
-    }
+    const int wpb_max = (int)max_wpb;
+    int wpb = resident_wpb(lfn, wave_bytes, block_extra, wpb_max);
     if (wpb < 1) {   // joined below before the error is returned
       lerr = fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
       break;
     }
     const auto tu = ctx->tune.find(m->etype);
-    if (tu != ctx->tune.end() && tu->second.wpb > 0 && (size_t)tu->second.wpb <= max_wpb &&
-        (size_t)tu->second.wpb * wave_bytes + block_extra <= kLdsPerCU)
-      wpb = tu->second.wpb;
+    int want_wpb = (tu != ctx->tune.end()) ? tu->second.wpb : 0;
+    if (const char* env = std::getenv("LLFEM_PANEL_WPB")) want_wpb = std::atoi(env);   // A/B knob
+    if (want_wpb > 0 && (size_t)want_wpb <= max_wpb && (size_t)want_wpb * wave_bytes + block_extra <= kLdsPerCU)
+      wpb = want_wpb;
     const int64_t used = (L.waves + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
     const size_t lds_bytes = wave_bytes * wpb + block_extra;

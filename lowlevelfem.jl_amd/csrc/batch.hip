@@ -513,8 +513,8 @@ extern "C" int llfem_autotune(llfem_mesh* m, const llfem_op_desc* d, llfem_tune_
   llfem_ctx* ctx = m->ctx;
   const int et = m->etype;
   const Tune saved = ctx->tune.count(et) ? ctx->tune[et] : Tune{};
-  auto time_shape = [&](int mult, int wpb, double* ms) -> int {
-    ctx->tune[et] = Tune{mult, wpb, 0.0};
+  auto time_shape = [&](const Tune& t, double* ms) -> int {
+    ctx->tune[et] = t;
     llfem_stats s{};
     LLFEM_TRY(llfem_assemble(m, d, nullptr, &s));   // warm-up
     double best = 1e30;
@@ -527,7 +527,7 @@ extern "C" int llfem_autotune(llfem_mesh* m, const llfem_op_desc* d, llfem_tune_
   };
   const int mults[] = {1, 2, 4, 8, 16, 32};
   const int wpbs[] = {0, 1, 2, 4, 8};
-  Tune best{8, 0, 1e30};
+  Tune best{kPanelMult, 0, 1e30};
   double default_ms = 0.0;
   int cand = 0;
   for (int mult : mults) {
@@ -536,15 +536,33 @@ extern "C" int llfem_autotune(llfem_mesh* m, const llfem_op_desc* d, llfem_tune_
     if (rc != LLFEM_OK) { ctx->tune[et] = saved; return rc; }
     for (int wpb : wpbs) {
       double ms = 0.0;
-      rc = time_shape(mult, wpb, &ms);
+      rc = time_shape(Tune{mult, wpb, 0.0}, &ms);
       if (rc != LLFEM_OK) { ctx->tune[et] = saved; rebuild_records(m); return rc; }
       ++cand;
-      if (mult == 8 && wpb == 0) default_ms = ms;
+      if (mult == kPanelMult && wpb == 0) default_ms = ms;
       if (ms < best.ms) best = Tune{mult, wpb, ms};
     }
   }
+  // the best shape, then the class launches' stream count x round kernels
+  // (the device time of an assembly spans every stream's launches)
   ctx->tune[et] = best;
   LLFEM_TRY(rebuild_records(m));
+  for (int streams = 1; streams <= 3; ++streams) {
+    for (int rounds = 1; rounds >= 0; --rounds) {
+      Tune t = best;
+      t.streams = streams;
+      t.rounds = rounds;
+      double ms = 0.0;
+      const int rc = time_shape(t, &ms);
+      if (rc != LLFEM_OK) { ctx->tune[et] = saved; rebuild_records(m); return rc; }
+      ++cand;
+      if (ms < best.ms) {
+        t.ms = ms;
+        best = t;
+      }
+    }
+  }
+  ctx->tune[et] = best;
   if (out) {
     out->elem_type = et;
     out->mult = best.mult;
@@ -552,6 +570,8 @@ extern "C" int llfem_autotune(llfem_mesh* m, const llfem_op_desc* d, llfem_tune_
     out->candidates = cand;
     out->best_ms = best.ms;
     out->default_ms = default_ms;
+    out->streams = best.streams > 0 ? best.streams : 2;
+    out->round_kernels = best.rounds == 0 ? 0 : 1;
   }
   return LLFEM_OK;
 }

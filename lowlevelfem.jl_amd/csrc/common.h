@@ -70,12 +70,18 @@ struct RuleState {
 };
 
 // Tuned panel parameters of one element type (llfem_autotune): launch width
-// multiplier of the record planner (0 = default) and waves per block (0 =
-// the most resident waves).
+// multiplier of the record planner (0 = kPanelMult), waves per block (0 = the
+// most resident waves), streams the node-class launches are spread over (0 =
+// default two) and whether the step-unrolled round kernels run where they
+// exist (-1 = default yes, 0 = no, 1 = yes).
+constexpr int kPanelMult = 4;   // default launch-width multiplier of the record planner
+
 struct Tune {
   int32_t mult = 0;
   int32_t wpb = 0;
   double ms = 0.0;
+  int32_t streams = 0;
+  int32_t rounds = -1;
 };
 
 struct AmgHier;

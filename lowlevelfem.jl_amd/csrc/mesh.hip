@@ -19,6 +19,8 @@
 #include <memory>
 #include <thread>
 
+#include <cstdlib>
+
 #include "common.h"
 
 namespace llfem {
@@ -235,9 +237,11 @@ int build_records(llfem_mesh* m, const std::vector<int32_t>& h_inc_ptr) {
     const int blk = 9;   // LDS sized for stiffness; mass uses less
     const int64_t wave_bytes = 8 * ((int64_t)npw * rmax * blk + 9);
     const int64_t wpc = std::max<int64_t>(1, std::min<int64_t>(16, (160 * 1024) / wave_bytes));
-    // wavefronts per launch: mult x resident waves (8 unless tuned)
+    // wavefronts per launch: mult x resident waves (kPanelMult unless tuned;
+    // LLFEM_PANEL_MULT=n sets it for A/B runs)
     const auto tu = ctx->tune.find(m->etype);
-    const int64_t mult = (tu != ctx->tune.end() && tu->second.mult > 0) ? tu->second.mult : 8;
+    int64_t mult = (tu != ctx->tune.end() && tu->second.mult > 0) ? tu->second.mult : kPanelMult;
+    if (const char* env = std::getenv("LLFEM_PANEL_MULT")) mult = std::max(1, std::min(64, std::atoi(env)));
     const int64_t target = mult * ncu * wpc;
     int64_t G = (nodes + npw * target - 1) / (npw * target);
     G = std::max<int64_t>(1, std::min<int64_t>(G, 256));

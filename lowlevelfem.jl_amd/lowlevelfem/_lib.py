@@ -51,7 +51,8 @@ class Stats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("elem_type", C.c_int32), ("mult", C.c_int32), ("waves_per_block", C.c_int32),
-                ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+                ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("streams", C.c_int32), ("round_kernels", C.c_int32)]
 
 
 class RequestSpec(C.Structure):   # llfem_request

@@ -113,9 +113,11 @@ def test_autotune_keeps_results(gpu_ctx):
     d = _dev(gpu_ctx, m)
     before, _ = d.assemble(_lib.OP_STIFFNESS, tab)
     res = d.autotune(_lib.OP_STIFFNESS, tab)
-    say(f"  autotune tet10: mult {res.mult} wpb {res.waves_per_block} best {res.best_ms:.4f} ms "
-        f"default {res.default_ms:.4f} ms over {res.candidates} shapes")
-    assert res.candidates == 30 and res.best_ms <= res.default_ms
+    say(f"  autotune tet10: mult {res.mult} wpb {res.waves_per_block} streams {res.streams} "
+        f"rounds {res.round_kernels} best {res.best_ms:.4f} ms default {res.default_ms:.4f} ms "
+        f"over {res.candidates} configurations")
+    assert res.candidates == 36 and res.best_ms <= res.default_ms
+    assert 1 <= res.streams <= 3 and res.round_kernels in (0, 1)
     after, _ = d.assemble(_lib.OP_STIFFNESS, tab)
     assert np.abs(after - before).max() <= 1e-14 * np.abs(before).max()
     # a mesh created afterwards on the same context uses the tuned shape
