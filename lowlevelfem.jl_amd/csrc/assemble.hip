@@ -493,11 +493,18 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
       !std::getenv("LLFEM_PANEL_NO_ROUNDS") && !(tuned != ctx->tune.end() && tuned->second.rounds == 0);
   nstreams = std::min<int>(nstreams, (int)plan.size());
   std::vector<int> stream_of(plan.size(), 0);
+  // issue order: largest work first, so the longest launch starts at once on
+  // its stream instead of queueing behind a shorter one (LLFEM_PANEL_ORDER=plan
+  // keeps plan order, for A/B runs)
+  std::vector<size_t> issue(plan.size());
+  std::iota(issue.begin(), issue.end(), size_t(0));
   if (nstreams > 1) {
     std::vector<size_t> by_work(plan.size());
     std::iota(by_work.begin(), by_work.end(), size_t(0));
     auto work = [&](size_t i) { return plan[i].waves * (int64_t)plan[i].rounds * plan[i].cnt; };
     std::stable_sort(by_work.begin(), by_work.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+    const char* ord = std::getenv("LLFEM_PANEL_ORDER");
+    if (!(ord && std::strcmp(ord, "plan") == 0)) issue = by_work;
     std::vector<int64_t> load(nstreams, 0);
     for (size_t i : by_work) {
       const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
@@ -508,7 +515,7 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     for (int k = 0; k + 1 < nstreams; ++k) LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
   }
   int lerr = LLFEM_OK;
-  for (size_t li = 0; li < plan.size(); ++li) {
+  for (size_t li : issue) {
     const PanelLaunch& L = plan[li];
     hipStream_t ls = stream_of[li] == 0 ? st : ctx->side[stream_of[li] - 1];
     P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
