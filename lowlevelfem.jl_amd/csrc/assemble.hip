@@ -132,30 +132,42 @@ PanelFn select_rounds_real(int npe, int cnt, int ks_needed) {
 // the runtime's occupancy calculator allows for the kernel's registers.  The
 // round kernels use up to ~240 VGPRs (two waves per SIMD): sized by LDS alone,
 // a 6-wave block left a CU with one block, 6 of its 8 wave slots.
-// Cached per (kernel, sizes).  LLFEM_PANEL_OCC=0 sizes by LDS alone (A/B).
-int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb) {
-  static std::mutex mu;
-  static std::map<std::tuple<const void*, size_t, size_t, int>, int> cache;
+// LLFEM_PANEL_OCC=0 sizes by LDS alone (A/B).
+bool occupancy_sizing() {
   const char* env = std::getenv("LLFEM_PANEL_OCC");
-  const bool occ = !env || std::atoi(env) != 0;
-  const auto key = std::make_tuple((const void*)fn, wave_bytes, block_extra, max_wpb * (occ ? 1 : -1));
+  return !env || std::atoi(env) != 0;
+}
+
+// Waves of `fn` resident on a CU with c-wave blocks (LDS, and registers
+// through the occupancy calculator unless LLFEM_PANEL_OCC=0).  Cached.
+size_t resident_waves(PanelFn fn, size_t wave_bytes, size_t block_extra, int c) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, size_t, size_t, int>, size_t> cache;
+  const bool occ = occupancy_sizing();
+  const auto key = std::make_tuple((const void*)fn, wave_bytes, block_extra, c * (occ ? 1 : -1));
   {
     std::lock_guard<std::mutex> lk(mu);
     const auto it = cache.find(key);
     if (it != cache.end()) return it->second;
   }
-  if (occ) (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCU);
+  const size_t lds = c * wave_bytes + block_extra;
+  size_t blocks = lds <= kLdsPerCU ? kLdsPerCU / lds : 0;
+  if (occ && blocks > 0) {
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCU);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, 64 * c, lds) == hipSuccess && nb > 0)
+      blocks = std::min(blocks, (size_t)nb);
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  return cache[key] = (size_t)c * blocks;
+}
+
+int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb) {
+  const bool occ = occupancy_sizing();
   int wpb = 0;
   size_t best = 0;
   for (int c = 1; c <= max_wpb && c * wave_bytes + block_extra <= kLdsPerCU; ++c) {
-    const size_t lds = c * wave_bytes + block_extra;
-    size_t blocks = kLdsPerCU / lds;
-    if (occ) {
-      int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)fn, 64 * c, lds) == hipSuccess && nb > 0)
-        blocks = std::min(blocks, (size_t)nb);
-    }
-    const size_t resident = (size_t)c * blocks;
+    const size_t resident = resident_waves(fn, wave_bytes, block_extra, c);
     // ties: with a per-block table in LDS, the block nearest 4 waves (1-wave
     // blocks repeat the table fill per wave, 8-wave blocks leave a CU idle
     // while their last wave finishes); without one, the smallest block
@@ -166,8 +178,6 @@ int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb)
       wpb = c;
     }
   }
-  std::lock_guard<std::mutex> lk(mu);
-  cache[key] = wpb;
   return wpb;
 }
 
@@ -479,10 +489,9 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
                                  iso ? nullptr : P.C, target));
   else
     LLFEM_HIP(hipEventRecord(ctx->ev[1], st));
-  // Node classes write disjoint panels, so their launches may overlap: each
-  // launch goes to the least-loaded of two streams (largest work first;
-  // work = steps of the launch), which fills the tail of one class's launch
-  // with the others.  A tuned count, or LLFEM_PANEL_STREAMS=n (1..3), sets it.
+  // Node classes write disjoint panels, so their launches may overlap on two
+  // streams (a tuned count, or LLFEM_PANEL_STREAMS=n (1..3), sets it), which
+  // fills the tail of one class's launch with the others.
   static const std::vector<PanelLaunch> kNoLaunches;
   const std::vector<PanelLaunch>& plan = highorder ? kNoLaunches : m->launches;
   const auto tuned = ctx->tune.find(m->etype);
@@ -492,40 +501,19 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
   const bool rounds_on =
       !std::getenv("LLFEM_PANEL_NO_ROUNDS") && !(tuned != ctx->tune.end() && tuned->second.rounds == 0);
   nstreams = std::min<int>(nstreams, (int)plan.size());
-  std::vector<int> stream_of(plan.size(), 0);
-  // issue order: largest work first, so the longest launch starts at once on
-  // its stream instead of queueing behind a shorter one (LLFEM_PANEL_ORDER=plan
-  // keeps plan order, for A/B runs)
-  std::vector<size_t> issue(plan.size());
-  std::iota(issue.begin(), issue.end(), size_t(0));
-  if (nstreams > 1) {
-    std::vector<size_t> by_work(plan.size());
-    std::iota(by_work.begin(), by_work.end(), size_t(0));
-    auto work = [&](size_t i) { return plan[i].waves * (int64_t)plan[i].rounds * plan[i].cnt; };
-    std::stable_sort(by_work.begin(), by_work.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
-    const char* ord = std::getenv("LLFEM_PANEL_ORDER");
-    if (!(ord && std::strcmp(ord, "plan") == 0)) issue = by_work;
-    std::vector<int64_t> load(nstreams, 0);
-    for (size_t i : by_work) {
-      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      stream_of[i] = k;
-      load[k] += work(i);
-    }
-    LLFEM_HIP(hipEventRecord(ctx->fork_ev, st));   // after the geometry
-    for (int k = 0; k + 1 < nstreams; ++k) LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
-  }
+  // per launch: kernel, waves per block, LDS, grid, and resident waves per CU
+  struct Cfg {
+    PanelFn fn;
+    int wpb;
+    size_t lds_bytes;
+    unsigned blocks;
+    double cost;   // steps / resident waves per CU: the launch's time alone, in relative units
+  };
+  std::vector<Cfg> cfg(plan.size());
   int lerr = LLFEM_OK;
-  for (size_t li : issue) {
+  for (size_t li = 0; li < plan.size() && lerr == LLFEM_OK; ++li) {
     const PanelLaunch& L = plan[li];
-    hipStream_t ls = stream_of[li] == 0 ? st : ctx->side[stream_of[li] - 1];
-    P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
-    P.desc = m->d_desc + L.desc_base;
-    P.waves = L.waves;
-    P.rounds = L.rounds;
-    P.cnt = L.cnt;
-    P.max_r = L.max_r;
-    P.lds_per_wave = npw * L.max_r * blkl + 9;   // + dummy block
-    const size_t wave_bytes = sizeof(double) * (size_t)P.lds_per_wave;
+    const size_t wave_bytes = sizeof(double) * (size_t)(npw * L.max_r * blkl + 9);   // + dummy block
     size_t max_wpb = (iso || op == LLFEM_OP_MASS) ? 8 : 4;   // matches k_panel launch bounds
     if (const char* env = std::getenv("LLFEM_PANEL_MAX_WAVES"))     // tuning knob
       max_wpb = std::max<size_t>(1, std::min<size_t>(max_wpb, (size_t)std::atoi(env)));
@@ -543,36 +531,76 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     }
     // waves per block: the size that keeps the most waves resident on a CU
     // (several small blocks beat one large block with LDS left over)
-    const int wpb_max = (int)max_wpb;
-    int wpb = resident_wpb(lfn, wave_bytes, block_extra, wpb_max);
-    if (wpb < 1) {   // joined below before the error is returned
+    int wpb = resident_wpb(lfn, wave_bytes, block_extra, (int)max_wpb);
+    if (wpb < 1) {
       lerr = fail(LLFEM_EUNSUP, "llfem_assemble: node panel exceeds LDS");
       break;
     }
-    const auto tu = ctx->tune.find(m->etype);
-    int want_wpb = (tu != ctx->tune.end()) ? tu->second.wpb : 0;
+    int want_wpb = (tuned != ctx->tune.end()) ? tuned->second.wpb : 0;
     if (const char* env = std::getenv("LLFEM_PANEL_WPB")) want_wpb = std::atoi(env);   // A/B knob
     if (want_wpb > 0 && (size_t)want_wpb <= max_wpb && (size_t)want_wpb * wave_bytes + block_extra <= kLdsPerCU)
       wpb = want_wpb;
     const int64_t used = (L.waves + wpb - 1) / wpb;
-    const unsigned blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
-    const size_t lds_bytes = wave_bytes * wpb + block_extra;
-    if (lds_bytes > 65536) {
+    Cfg& c = cfg[li];
+    c.fn = lfn;
+    c.wpb = wpb;
+    c.blocks = (unsigned)((used + kXcds - 1) / kXcds * kXcds);   // see k_panel
+    c.lds_bytes = wave_bytes * wpb + block_extra;
+    const size_t res = std::max<size_t>(1, resident_waves(lfn, wave_bytes, block_extra, wpb));
+    c.cost = (double)L.waves * L.rounds * L.cnt / (double)res;
+    if (c.lds_bytes > 65536) {
       const hipError_t e = hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)lds_bytes);
-      if (e != hipSuccess) {   // joined below before the error is returned
-        lerr = fail(LLFEM_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-        break;
-      }
+                                               (int)c.lds_bytes);
+      if (e != hipSuccess) lerr = fail(LLFEM_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     }
-    hipLaunchKernelGGL(lfn, dim3(blocks), dim3(64 * wpb), lds_bytes, ls, P);
+  }
+  LLFEM_TRY(lerr);
+  // streams: each launch to the least-loaded stream, costliest first, and
+  // issued costliest first on its stream, so the longest launch starts at
+  // once instead of queueing behind a shorter one.  The cost counts the
+  // resident waves, so an LDS-bound class (fewer waves per CU) weighs more
+  // than its steps alone say.  LLFEM_PANEL_ORDER=plan issues in plan order
+  // and balances by steps (A/B).
+  std::vector<int> stream_of(plan.size(), 0);
+  std::vector<size_t> issue(plan.size());
+  std::iota(issue.begin(), issue.end(), size_t(0));
+  if (nstreams > 1) {
+    const char* ord = std::getenv("LLFEM_PANEL_ORDER");
+    const bool plan_order = ord && std::strcmp(ord, "plan") == 0;
+    auto cost = [&](size_t i) {
+      return plan_order ? (double)plan[i].waves * plan[i].rounds * plan[i].cnt : cfg[i].cost;
+    };
+    std::vector<size_t> by_cost(plan.size());
+    std::iota(by_cost.begin(), by_cost.end(), size_t(0));
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+    if (!plan_order) issue = by_cost;
+    std::vector<double> load(nstreams, 0.0);
+    for (size_t i : by_cost) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      stream_of[i] = k;
+      load[k] += cost(i);
+    }
+    LLFEM_HIP(hipEventRecord(ctx->fork_ev, st));   // after the geometry
+    for (int k = 0; k + 1 < nstreams; ++k) LLFEM_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+  }
+  for (size_t li : issue) {
+    const PanelLaunch& L = plan[li];
+    const Cfg& c = cfg[li];
+    hipStream_t ls = stream_of[li] == 0 ? st : ctx->side[stream_of[li] - 1];
+    P.rec = static_cast<const uint8_t*>(m->d_rec) + L.rec_base * m->rec_bytes;
+    P.desc = m->d_desc + L.desc_base;
+    P.waves = L.waves;
+    P.rounds = L.rounds;
+    P.cnt = L.cnt;
+    P.max_r = L.max_r;
+    P.lds_per_wave = npw * L.max_r * blkl + 9;   // + dummy block
+    hipLaunchKernelGGL(c.fn, dim3(c.blocks), dim3(64 * c.wpb), c.lds_bytes, ls, P);
   }
   LLFEM_HIP(hipGetLastError());
   for (int k = 0; k + 1 < nstreams; ++k) {   // join
     LLFEM_HIP(hipEventRecord(ctx->join_ev[k], ctx->side[k]));
     LLFEM_HIP(hipStreamWaitEvent(st, ctx->join_ev[k], 0));
   }
-  LLFEM_TRY(lerr);
   if (!highorder) LLFEM_HIP(hipEventRecord(ctx->ev[2], st));
   if (acc) {   // resident += this call's values (group order: the earlier groups first)
     k_add_into<<<(unsigned)((nnz + 255) / 256), 256, 0, st>>>(nnz, acc_tmp, *outp);
