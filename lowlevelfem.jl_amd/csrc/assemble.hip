@@ -548,13 +548,16 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
     c.lds_bytes = wave_bytes * wpb + block_extra;
     const size_t res = std::max<size_t>(1, resident_waves(lfn, wave_bytes, block_extra, wpb));
     c.cost = (double)L.waves * L.rounds * L.cnt / (double)res;
-    if (c.lds_bytes > 65536) {
-      const hipError_t e = hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)c.lds_bytes);
-      if (e != hipSuccess) lerr = fail(LLFEM_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    }
   }
   LLFEM_TRY(lerr);
+  // dynamic LDS above 64 KB: allow the kernel a CU's whole LDS (a limit, not
+  // a reservation), so no later call lowers it below what another launch uses
+  for (const Cfg& c : cfg)
+    if (c.lds_bytes > 65536) {
+      const hipError_t e =
+          hipFuncSetAttribute((const void*)c.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCU);
+      if (e != hipSuccess) return fail(LLFEM_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    }
   // streams: each launch to the least-loaded stream, costliest first, and
   // issued costliest first on its stream, so the longest launch starts at
   // once instead of queueing behind a shorter one.  The cost counts the
