@@ -122,17 +122,7 @@ PanelFn select_rounds_real(int npe, int cnt, int ks_needed) {
   return nullptr;
 }
 
-// Reference table in LDS for the FP64 round kernels: each lane reads its 9
-// table values from LDS instead of through the texture-address unit, the
-// busiest unit of these kernels (FP64 assembly of the tet10 beam 2.09 -> 2.04
-// ms; the FP32 kernels measured 2 % slower with it and keep the global table).
-// LLFEM_PANEL_TABLE_LDS=0 switches it off for A/B runs.
-// Waves per block (1..max_wpb) that keep the most waves of `fn` resident on
-// a CU: by LDS (c waves take c * wave_bytes + block_extra bytes) and by what
-// the runtime's occupancy calculator allows for the kernel's registers.  The
-// round kernels use up to ~240 VGPRs (two waves per SIMD): sized by LDS alone,
-// a 6-wave block left a CU with one block, 6 of its 8 wave slots.
-// LLFEM_PANEL_OCC=0 sizes by LDS alone (A/B).
+// LLFEM_PANEL_OCC=0 sizes panel blocks by LDS alone (A/B).
 bool occupancy_sizing() {
   const char* env = std::getenv("LLFEM_PANEL_OCC");
   return !env || std::atoi(env) != 0;
@@ -162,6 +152,11 @@ size_t resident_waves(PanelFn fn, size_t wave_bytes, size_t block_extra, int c) 
   return cache[key] = (size_t)c * blocks;
 }
 
+// Waves per block (1..max_wpb) that keep the most waves of `fn` resident on
+// a CU: by LDS (c waves take c * wave_bytes + block_extra bytes) and by what
+// the runtime's occupancy calculator allows for the kernel's registers.  The
+// round kernels use up to ~240 VGPRs (two waves per SIMD): sized by LDS alone,
+// a 6-wave block left a CU with one block, 6 of its 8 wave slots.
 int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb) {
   const bool occ = occupancy_sizing();
   int wpb = 0;
@@ -181,6 +176,17 @@ int resident_wpb(PanelFn fn, size_t wave_bytes, size_t block_extra, int max_wpb)
   return wpb;
 }
 
+// Reference table in LDS for the round kernels: each lane reads its 9 table
+// values from LDS instead of through the texture-address unit, the busiest
+// unit of these kernels (FP64 assembly of the tet10 beam 2.09 -> 2.04 ms; the
+// FP32 kernels measured 2 % slower with it until the launches were sized by
+// registers too, 3.5 % faster since).  LLFEM_PANEL_TABLE_LDS=0 and
+// LLFEM_PANEL_TABLE_LDS_F32=0 switch it off for A/B runs.
+bool rounds_table_lds_f32() {
+  const char* e = std::getenv("LLFEM_PANEL_TABLE_LDS_F32");
+  return !e || std::atoi(e) != 0;
+}
+
 bool rounds_table_lds() {
   const char* e = std::getenv("LLFEM_PANEL_TABLE_LDS");
   return !e || std::atoi(e) != 0;
@@ -189,7 +195,9 @@ bool rounds_table_lds() {
 PanelFn select_rounds(int npe, int map_bytes, int op, bool affine, bool iso, bool fp32, int cnt, int max_r) {
   if (map_bytes != 1 || op != LLFEM_OP_STIFFNESS || !affine || !iso) return nullptr;
   const int need = (9 * max_r + 7 + 63) / 64;   // up to 7 doubles of window alignment
-  if (fp32) return select_rounds_real<float, false>(npe, cnt, need);
+  if (fp32)
+    return rounds_table_lds_f32() ? select_rounds_real<float, true>(npe, cnt, need)
+                                  : select_rounds_real<float, false>(npe, cnt, need);
   return rounds_table_lds() ? select_rounds_real<double, true>(npe, cnt, need)
                             : select_rounds_real<double, false>(npe, cnt, need);
 }
@@ -526,7 +534,8 @@ int assemble_impl(llfem_mesh* m, const llfem_op_desc* d, double* nzval_out, llfe
       if (rounds_on)                                    // A/B knob / tuned: the step-unrolled kernel
         if (PanelFn f = select_rounds(npe, m->map_bytes, op, affine, iso, fp32, L.cnt, L.max_r)) {
           lfn = f;
-          if (rounds_table_lds() && !fp32) block_extra = 8 + 10 * sizeof(double) * npe * npe;
+          if (!fp32 && rounds_table_lds()) block_extra = 8 + 10 * sizeof(double) * npe * npe;
+          if (fp32 && rounds_table_lds_f32()) block_extra = 8 + 12 * sizeof(float) * npe * npe;
         }
     }
     // waves per block: the size that keeps the most waves resident on a CU
