@@ -386,14 +386,25 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     alg_bytes += 24.0 * m->N + 4.0 * m->npe * m->E + 8.0 * m->nadj * blk;
     if (mp) alg_bytes += 24.0 * m->N + 4.0 * m->npe * m->E + 8.0 * m->nadj * 3;
   }
-  // waves per block: 4 unless the widest panel does not fit
+  // one wavefront per block: each wave takes work items from the queue on its
+  // own, and a CU refills a finished wave's slot at once (4-wave blocks: the
+  // persistent kernel 10.05 -> 9.25 ms on the 256-problem batch; 2-wave
+  // blocks 10.5 ms).  The kernel's registers allow two waves per SIMD.
   const size_t wave_bytes = sizeof(double) * max_lds;
   if (!bsegs.empty() && wave_bytes > 160 * 1024)
     return fail(LLFEM_EUNSUP, "llfem_assemble_batch: node panel exceeds LDS");
-  int wpb = 4;
+  int wpb = 1;
+  if (const char* env = std::getenv("LLFEM_BATCH_WPB")) wpb = std::max(1, std::min(4, std::atoi(env)));   // A/B
   while (wpb > 1 && wpb * wave_bytes > 160 * 1024) --wpb;
   const size_t lds_bytes = wave_bytes * wpb;
-  const int per_cu = std::max<int>(1, (int)((160 * 1024) / std::max<size_t>(lds_bytes, 1)));
+  int per_cu = std::max<int>(1, (int)((160 * 1024) / std::max<size_t>(lds_bytes, 1)));
+  {   // blocks resident per CU: LDS, and the kernel's registers
+    (void)hipFuncSetAttribute((const void*)k_panel_batch, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_panel_batch, 64 * wpb, lds_bytes) ==
+            hipSuccess && nb > 0)
+      per_cu = std::min(per_cu, nb);
+  }
   // work items: about 8 per resident wavefront, each `span` launch waves
   const int64_t resident = (int64_t)std::max(1, ctx->num_cus) * per_cu * wpb;
   const int64_t span = std::max<int64_t>(1, wtot / (8 * resident));
@@ -419,9 +430,7 @@ extern "C" int llfem_assemble_batch(int32_t n, llfem_mesh* const* meshes, const 
     LLFEM_HIP(hipMemcpyAsync(d_bsegs, bsegs.data(), bbytes, hipMemcpyHostToDevice, st));
     const int64_t want = (items_total + wpb - 1) / wpb;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per_cu, want));
-    if (lds_bytes > 65536)
-      LLFEM_HIP(hipFuncSetAttribute((const void*)k_panel_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_bytes));
+
     LLFEM_HIP(hipEventRecord(ctx->ev[0], st));
     k_geometry_batch<<<(unsigned)((etot + 255) / 256), 256, 0, st>>>(d_gsegs, (int)gsegs.size(), etot);
     k_zero_u32<<<1, 64, 0, st>>>(d_queue);
